@@ -1,0 +1,118 @@
+/* libpylians_hip: C ABI of the MI355X (gfx950) density-assignment -> P(k)/B(k) path.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero PYL_E_* code otherwise; the
+ *    message is available from pyl_last_error() (thread-local).
+ *  - Unless a name ends in a MAS_c.h drop-in (pyl_mas_NGP/CIC/TSC/PCS), pointers are DEVICE
+ *    pointers, the caller owns every input and output buffer, and the work is queued on
+ *    `stream` (a hipStream_t; NULL = the default stream) without host synchronisation.
+ *    The library owns FFT twiddle tables, window tables, bin-geometry tables and grow-only
+ *    workspaces, cached per device.
+ *  - Calls that share the library workspaces must not run concurrently on different streams
+ *    of one device.
+ *  - prec selects the working precision: PYL_F32 (float32 grid and FFT, float64 shell sums)
+ *    or PYL_F64 (float64 grid, FFT and sums).
+ */
+#ifndef PYLIANS_HIP_H
+#define PYLIANS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYL_ABI_VERSION 1
+
+#define PYL_OK 0
+#define PYL_E_ARG 1
+#define PYL_E_UNSUPPORTED 2
+#define PYL_E_MODES 3
+#define PYL_E_ALLOC 4
+#define PYL_E_LAUNCH 5
+
+/* mass-assignment schemes; also the window exponent p of Pk_library.MAS_function */
+#define PYL_NGP 1
+#define PYL_CIC 2
+#define PYL_TSC 3
+#define PYL_PCS 4
+
+#define PYL_F32 4
+#define PYL_F64 8
+
+/* deposit flags */
+#define PYL_GRID_F64 1   /* grid is float64 (CICW_d / NGPW_d) */
+#define PYL_DUP2D 2      /* 2-D Cython path: each weight deposited STENCIL times */
+#define PYL_RENORM2D 4   /* ... then the whole grid divided by STENCIL */
+#define PYL_ZERO 8       /* zero the grid first (MA itself accumulates) */
+#define PYL_ATOMIC 16    /* force the direct global-atomic deposit */
+#define PYL_BINNED 32    /* force the cell-binned LDS deposit */
+
+/* Pk flags */
+#define PYL_PK_1D 1
+#define PYL_PK_2D 2
+#define PYL_PK_PHASE 4
+#define PYL_PK_ALL 7
+
+int pyl_abi_version(void);
+const char* pyl_last_error(void);
+
+/* ---------------------------------------------------------------- mass assignment
+ * Replaces MAS_library.MA and the per-scheme Cython functions NGP/CIC/TSC/PCS, NGPW/CICW/TSCW/PCSW,
+ * CICW_d/NGPW_d (MAS_library.pyx:57-545).  pos: (n, axes) float32 row-major; W: (n,) float32 or
+ * NULL; grid: dims^axes float32 (float64 with PYL_GRID_F64), accumulated into (MA never zeroes,
+ * MAS_library.pyx:159-166).  For the 2-D Cython semantics pass PYL_DUP2D [| PYL_RENORM2D]. */
+int pyl_mas_deposit(const float* pos, const float* W, int64_t n, int axes, int dims, double BoxSize,
+                    int scheme, int flags, void* grid, void* stream);
+
+/* Host-pointer drop-ins of the OpenMP C kernels, same argument list as MAS_c.h:3-10
+ * (void NGP|CIC|TSC|PCS(FLOAT *pos, FLOAT *number, FLOAT *W, long particles, int dims, int axes,
+ * FLOAT BoxSize, int threads)).  Synchronous; `threads` is accepted and ignored. */
+int pyl_mas_NGP(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                int threads);
+int pyl_mas_CIC(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                int threads);
+int pyl_mas_TSC(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                int threads);
+int pyl_mas_PCS(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                int threads);
+
+/* ---------------------------------------------------------------- FFT
+ * Replace FFT3Dr_f/_d (unnormalised r2c, (N,N,N) -> (N,N,N/2+1)) and IFFT3Dr_f/_d (c2r,
+ * 1/N^3-normalised when normalise != 0), Pk_library.pyx:120-181.  The input of pyl_fft_c2r is
+ * not modified. */
+int pyl_fft_r2c(const void* in, void* out, int dims, int prec, void* stream);
+int pyl_fft_c2r(const void* in, void* out, int dims, int prec, int normalise, void* stream);
+
+/* ---------------------------------------------------------------- power spectrum
+ * Output layout of pyl_pk / pyl_pk_particles (float64, out_len values):
+ *   k3D[kmax] P0[kmax] P2[kmax] P4[kmax] Pkphase[kmax] Nmodes3D[kmax]
+ *   k1D[kmax_par] Pk1D[kmax_par] Nmodes1D[kmax_par]
+ *   kpar[n2] kper[n2] Pk2D[n2] Nmodes2D[n2],   n2 = (kmax_par+1)*(kmax_per+1)
+ * i.e. the attributes of Pk_library.Pk (Pk_library.pyx:387-425). */
+int pyl_pk_layout(int dims, int* kmax_par, int* kmax_per, int* kmax, int64_t* out_len);
+
+/* Replaces Pk_library.Pk (Pk_library.pyx:266-425).  delta: (N,N,N) real of precision prec.
+ * mas_p: window exponent (0 = no deconvolution, as MAS='None').  deltak_out (nullable) receives
+ * the corrected half spectrum (keep_deltak, Pk_library.pyx:423).  Returns PYL_E_MODES when the
+ * independent-mode count is not (N^3-8)/2+8 (check_number_modes, Pk_library.pyx:90-102). */
+int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags, double* out,
+           void* deltak_out, void* stream);
+
+/* Fused end-to-end path of Pk_snapshot.Pk_comp (Pk_snapshot.py:80-91): deposit with `scheme`,
+ * delta = rho/rhobar - 1 with rhobar = sum(W)/N^3 (N_p/N^3 without weights), r2c, window
+ * deconvolution with the same scheme, binning.  The density grid is not returned. */
+int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
+                     int prec, int flags, double* out, void* stream);
+
+/* ---------------------------------------------------------------- bispectrum
+ * Replaces Pk_library.Bk (bispectrum_library.pyx:32-196).  theta: HOST array of ntheta angles.
+ * B, Q (ntheta) and k, Pk (ntheta+2) are HOST float64 arrays; the call synchronises `stream`. */
+int pyl_bk(const void* delta, int dims, double BoxSize, double k1, double k2, const double* theta, int ntheta,
+           int mas_p, int prec, double* B, double* Q, double* k, double* Pk, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYLIANS_HIP_H */

@@ -1,0 +1,191 @@
+// FFT-shell bispectrum (Pk_library.Bk, bispectrum_library.pyx:32-196) on gfx950.
+//
+// delta_k = r2c(delta); for every shell i (k1, k2, k3(theta_j)) the masked spectra
+//   D_i(k) = W(k) delta_k and I_i(k) = 1 on kmin_i <= |k| < kmax_i (kF units, every half-spectrum
+//   mode: the independence cut is commented out in the reference, :100-105)
+// are inverse transformed with 1/N^3 normalisation.  The shell mask and the window are applied
+// inside the first inverse pass (no masked spectrum is materialised) and the float64 reductions
+// sum d_i^2, sum I_i^2, sum d1 d2 d3, sum I1 I2 I3 are fused into the last (c2r rows) pass, which
+// multiplies by the stored d1*d2 (resp. I1*I2) field instead of writing d3.
+#include <math.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/pylians_hip.h"
+#include "fft.hpp"
+#include "fft_engine.hpp"
+#include "util.hpp"
+
+namespace pyl {
+
+const double* pk_window_table(int N, int p);
+
+template <typename R, int N, int NL, int T>
+__global__ __launch_bounds__(T) void k_shell_x_inv(const cplx<R>* __restrict__ spec, cplx<R>* __restrict__ dst,
+                                                    long long P, double kmin, double kmax, int indicator,
+                                                    const double* __restrict__ wing,
+                                                    const cplx<R>* __restrict__ twg) {
+    __shared__ cplx<R> s[N * NL];
+    __shared__ cplx<R> tw[N];
+    __shared__ double win[N];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N; i += T) {
+        tw[i] = twg[i];
+        win[i] = wing[i];
+    }
+    __syncthreads();
+    const long long ncols = (long long)N * P;
+    const int l = tid % NL;
+    const long long c = (long long)blockIdx.x * NL + l;
+    const bool valid = c < ncols;
+    const int kyy = valid ? (int)(c / P) : 0, kzz = valid ? (int)(c % P) : 0;
+    const int ky = kfreq(kyy, N), kz = kzz;
+    const double cyz = win[kyy] * win[kzz];
+    for (int j = tid / NL; j < N; j += T / NL) {
+        const int kx = kfreq(j, N);
+        const double k = sqrt((double)(kx * kx + ky * ky + kz * kz));
+        cplx<R> v = mk<R>(0, 0);
+        if (valid && k >= kmin && k < kmax) {
+            if (indicator) {
+                v = mk<R>(1, 0);
+            } else {
+                const R fac = (R)(win[j] * cyz);
+                v = scale(spec[c + j * ncols], fac);
+            }
+        }
+        s[j * NL + l] = conjg(v);
+    }
+    __syncthreads();
+    lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
+    if (valid)
+        for (int j = tid / NL; j < N; j += T / NL) dst[c + j * ncols] = conjg(s[j * NL + l]);
+}
+
+template <typename R, int N>
+static int launch_shell(const cplx<R>* spec, cplx<R>* dst, int64_t P, double kmin, double kmax, int indicator,
+                        const double* win, hipStream_t st) {
+    constexpr int NL = Geo<R>::NL;
+    constexpr int T = threads_for(N, NL);
+    const cplx<R>* tw = twiddles<R>(N);
+    PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
+    const long long ncols = (long long)N * P;
+    hipLaunchKernelGGL((k_shell_x_inv<R, N, NL, T>), dim3((unsigned)((ncols + NL - 1) / NL)), dim3(T), 0, st, spec,
+                       dst, (long long)P, kmin, kmax, indicator, win, tw);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+#define PYL_BK_SIZES(X) \
+    X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
+    X(320) X(384) X(512) X(640) X(768) X(1024)
+
+template <typename R>
+static int shell_x_inv(const cplx<R>* spec, cplx<R>* dst, int N, int64_t P, double kmin, double kmax, int indicator,
+                       const double* win, hipStream_t st) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_shell<R, n>(spec, dst, P, kmin, kmax, indicator, win, st);
+        PYL_BK_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("grid size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+// One shell field through the inverse transform: first pass masked, then y, then rows with the
+// fused reductions / stores.
+template <typename R>
+static int shell_field(const cplx<R>* spec, cplx<R>* ws, int N, int64_t P, double kmin, double kmax, int indicator,
+                       const double* win, R* out, int store, const R* aux, double* acc, hipStream_t st) {
+    int rc = shell_x_inv<R>(spec, ws, N, P, kmin, kmax, indicator, win, st);
+    if (rc) return rc;
+    rc = fft_pass_cols<R>(ws, ws, N, P, 1, true, (R)1, st);
+    if (rc) return rc;
+    const R sc = (R)(1.0 / ((double)N * N * N));
+    return fft_pass_z_c2r<R>(ws, out, N, P, sc, store, aux, acc, st);
+}
+
+template <typename R>
+static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, const double* theta, int nb, int mas_p,
+                  double* B, double* Q, double* kall, double* Pk, hipStream_t st) {
+    int rc = OK;
+    const int64_t P = half_len(N);
+    const size_t nspec = (size_t)N * N * P, ncell = (size_t)N * N * N;
+    const double kF = 2.0 * M_PI / BoxSize;
+    std::vector<double> kmin(nb + 2), kmax(nb + 2);
+    kall[0] = k1;
+    kall[1] = k2;
+    for (int j = 0; j < nb; ++j) {
+        const double s = k2 * sin(theta[j]), c = k2 * cos(theta[j]) + k1;
+        kall[j + 2] = sqrt(s * s + c * c);
+    }
+    for (int i = 0; i < nb + 2; ++i) {
+        kmin[i] = (kall[i] - kF) / kF;
+        kmax[i] = (kall[i] + kF) / kF;
+    }
+    const double* win = pk_window_table(N, mas_p);
+    PYL_CHECK(win, E_ALLOC, "window table allocation failed");
+    cplx<R>* spec = (cplx<R>*)workspace(WS_SPEC, sizeof(cplx<R>) * nspec, &rc);
+    if (rc) return rc;
+    cplx<R>* ws = (cplx<R>*)workspace(WS_SPEC2, sizeof(cplx<R>) * nspec, &rc);
+    if (rc) return rc;
+    R* f = (R*)workspace(WS_AUX, sizeof(R) * ncell * 4, &rc);
+    if (rc) return rc;
+    R *d1 = f, *d12 = f + ncell, *I1 = f + 2 * ncell, *I12 = f + 3 * ncell;
+    const size_t nacc = 4 * (size_t)(nb + 2);
+    double* acc = (double*)workspace(WS_SUMS, sizeof(double) * nacc, &rc);
+    if (rc) return rc;
+    rc = zero_fill(acc, sizeof(double) * nacc, st);
+    if (rc) return rc;
+    rc = fft3d_r2c<R>(delta, spec, N, P, st);
+    if (rc) return rc;
+    // shell k1: d1, I1
+    if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 0, win, d1, 1, nullptr, acc + 0, st))) return rc;
+    if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 1, win, I1, 1, nullptr, acc + 2, st))) return rc;
+    // shell k2: store d1*d2 and I1*I2
+    if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 0, win, d12, 2, d1, acc + 4, st))) return rc;
+    if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
+    for (int j = 0; j < nb; ++j) {
+        double* a = acc + 4 * (j + 2);
+        if ((rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 0, win, nullptr, 0, d12, a, st))) return rc;
+        if ((rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 1, win, nullptr, 0, I12, a + 2, st)))
+            return rc;
+    }
+    std::vector<double> h(nacc);
+    PYL_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st));
+    PYL_HIP(hipStreamSynchronize(st));
+    const double unitP = pow(BoxSize / ((double)N * N), 3);
+    const double unitB = pow(BoxSize * BoxSize / ((double)N * N * N), 3);
+    for (int i = 0; i < nb + 2; ++i) {
+        PYL_CHECK(h[4 * i + 2] != 0.0, E_ARG, "float division by zero: empty shell %d", i);
+        Pk[i] = (h[4 * i + 0] / h[4 * i + 2]) * unitP;
+    }
+    for (int j = 0; j < nb; ++j) {
+        const double tri = h[4 * (j + 2) + 3];
+        PYL_CHECK(tri != 0.0, E_ARG, "float division by zero: no closed triangles for theta bin %d", j);
+        B[j] = (h[4 * (j + 2) + 1] / tri) * unitB;
+        Q[j] = B[j] / (Pk[0] * Pk[1] + Pk[0] * Pk[j + 2] + Pk[1] * Pk[j + 2]);
+    }
+    return OK;
+}
+
+}  // namespace pyl
+
+using namespace pyl;
+
+extern "C" int pyl_bk(const void* delta, int dims, double BoxSize, double k1, double k2, const double* theta,
+                      int ntheta, int mas_p, int prec, double* B, double* Q, double* k, double* Pk, void* stream) {
+    PYL_CHECK(prec == PYL_F32 || prec == PYL_F64, E_ARG, "prec must be 4 or 8");
+    PYL_CHECK(fft_size_supported(dims), E_UNSUPPORTED, "grid size %d not supported", dims);
+    PYL_CHECK(ntheta >= 0 && (ntheta == 0 || theta), E_ARG, "bad theta array");
+    PYL_CHECK(delta && B && Q && k && Pk, E_ARG, "null pointer");
+    PYL_CHECK(mas_p >= 0 && mas_p <= 4, E_ARG, "window exponent must be in [0, 4]");
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == PYL_F32)
+        return bk_run<float>((const float*)delta, dims, BoxSize, k1, k2, theta, ntheta, mas_p, B, Q, k, Pk, st);
+    return bk_run<double>((const double*)delta, dims, BoxSize, k1, k2, theta, ntheta, mas_p, B, Q, k, Pk, st);
+}

@@ -1,0 +1,215 @@
+// C ABI of libpylians_hip (declared in include/pylians_hip.h).
+#include <math.h>
+#include <string.h>
+
+#include "../../include/pylians_hip.h"
+#include "common.hpp"
+#include "deposit.hpp"
+#include "fft.hpp"
+#include "pk.hpp"
+#include "util.hpp"
+
+using namespace pyl;
+
+static_assert(PYL_GRID_F64 == DEP_GRID_F64 && PYL_DUP2D == DEP_DUP2D && PYL_RENORM2D == DEP_RENORM2D &&
+                  PYL_ZERO == DEP_ZERO && PYL_ATOMIC == DEP_ATOMIC && PYL_BINNED == DEP_BINNED,
+              "flag mismatch");
+static_assert(PYL_PK_1D == PK_1D && PYL_PK_2D == PK_2D && PYL_PK_PHASE == PK_PHASE, "flag mismatch");
+
+namespace pyl {
+int weight_sum(const float* W, int64_t n, double* out, hipStream_t st);
+}
+
+extern "C" int pyl_abi_version(void) { return PYL_ABI_VERSION; }
+extern "C" const char* pyl_last_error(void) { return last_error(); }
+
+// ------------------------------------------------------------------ mass assignment
+extern "C" int pyl_mas_deposit(const float* pos, const float* W, int64_t n, int axes, int dims, double BoxSize,
+                               int scheme, int flags, void* grid, void* stream) {
+    PYL_CHECK(n >= 0, E_ARG, "negative particle count");
+    PYL_CHECK(n == 0 || pos, E_ARG, "null positions");
+    PYL_CHECK(grid, E_ARG, "null grid");
+    return deposit(pos, W, n, axes, dims, BoxSize, scheme, flags, grid, (hipStream_t)stream);
+}
+
+static int mas_c_host(int scheme, const float* pos, float* number, const float* W, long particles, int dims,
+                      int axes, float BoxSize) {
+    PYL_CHECK(particles >= 0 && dims > 0 && (axes == 2 || axes == 3), E_ARG, "bad arguments");
+    PYL_CHECK(pos && number, E_ARG, "null pointer");
+    size_t cells = (size_t)dims * dims * (axes == 3 ? dims : 1);
+    int rc = OK;
+    float* dpos = (float*)workspace(WS_HOST_POS, sizeof(float) * (size_t)particles * axes + 16, &rc);
+    if (rc) return rc;
+    float* dgrid = (float*)workspace(WS_HOST_GRID, sizeof(float) * cells, &rc);
+    if (rc) return rc;
+    float* dW = nullptr;
+    if (W) {
+        dW = (float*)workspace(WS_HOST_W, sizeof(float) * (size_t)particles + 16, &rc);
+        if (rc) return rc;
+    }
+    PYL_HIP(hipMemcpy(dpos, pos, sizeof(float) * (size_t)particles * axes, hipMemcpyHostToDevice));
+    PYL_HIP(hipMemcpy(dgrid, number, sizeof(float) * cells, hipMemcpyHostToDevice));
+    if (W) PYL_HIP(hipMemcpy(dW, W, sizeof(float) * (size_t)particles, hipMemcpyHostToDevice));
+    // MAS_c.c 2-D path deposits every weight once (n_max = 1): no DUP2D
+    rc = deposit(dpos, dW, particles, axes, dims, (double)BoxSize, scheme, 0, dgrid, 0);
+    if (rc) return rc;
+    PYL_HIP(hipMemcpy(number, dgrid, sizeof(float) * cells, hipMemcpyDeviceToHost));
+    return OK;
+}
+
+extern "C" int pyl_mas_NGP(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                           int threads) {
+    (void)threads;
+    return mas_c_host(1, pos, number, W, particles, dims, axes, BoxSize);
+}
+extern "C" int pyl_mas_CIC(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                           int threads) {
+    (void)threads;
+    return mas_c_host(2, pos, number, W, particles, dims, axes, BoxSize);
+}
+extern "C" int pyl_mas_TSC(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                           int threads) {
+    (void)threads;
+    return mas_c_host(3, pos, number, W, particles, dims, axes, BoxSize);
+}
+extern "C" int pyl_mas_PCS(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
+                           int threads) {
+    (void)threads;
+    return mas_c_host(4, pos, number, W, particles, dims, axes, BoxSize);
+}
+
+// ------------------------------------------------------------------ FFT
+static int check_fft(int dims, int prec) {
+    PYL_CHECK(prec == PYL_F32 || prec == PYL_F64, E_ARG, "prec must be 4 or 8");
+    PYL_CHECK(fft_size_supported(dims), E_UNSUPPORTED,
+              "grid size %d not supported (supported: 2^a 3^b 5^c, 8 <= N <= 1024, from the built-in list)", dims);
+    return OK;
+}
+
+extern "C" int pyl_fft_r2c(const void* in, void* out, int dims, int prec, void* stream) {
+    int rc = check_fft(dims, prec);
+    if (rc) return rc;
+    const int64_t P = half_len(dims);
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == PYL_F32) return fft3d_r2c<float>((const float*)in, (cplx<float>*)out, dims, P, st);
+    return fft3d_r2c<double>((const double*)in, (cplx<double>*)out, dims, P, st);
+}
+
+extern "C" int pyl_fft_c2r(const void* in, void* out, int dims, int prec, int normalise, void* stream) {
+    int rc = check_fft(dims, prec);
+    if (rc) return rc;
+    const int64_t P = half_len(dims);
+    const size_t nspec = (size_t)dims * dims * P;
+    hipStream_t st = (hipStream_t)stream;
+    const double sc = normalise ? 1.0 / ((double)dims * dims * dims) : 1.0;
+    void* ws = workspace(WS_SPEC2, nspec * (prec == PYL_F32 ? 8 : 16), &rc);
+    if (rc) return rc;
+    if (prec == PYL_F32)
+        return fft3d_c2r<float>((const cplx<float>*)in, (float*)out, (cplx<float>*)ws, dims, P, (float)sc, st);
+    return fft3d_c2r<double>((const cplx<double>*)in, (double*)out, (cplx<double>*)ws, dims, P, sc, st);
+}
+
+// ------------------------------------------------------------------ power spectrum
+extern "C" int pyl_pk_layout(int dims, int* kmax_par, int* kmax_per, int* kmax, int64_t* out_len) {
+    PYL_CHECK(dims > 1, E_ARG, "dims must be > 1");
+    PkDims d = pk_dims(dims, half_len(dims));
+    if (kmax_par) *kmax_par = d.kmax_par;
+    if (kmax_per) *kmax_per = d.kmax_per;
+    if (kmax) *kmax = d.kmax;
+    if (out_len) *out_len = (int64_t)pk_out_len(d);
+    return OK;
+}
+
+// float64 sums live in one workspace: s3 (4*n3) | s1 (n1) | s2 (n2) | mass (2)
+static int pk_sums_ws(const PkDims& d, PkSums* s, double** mass, hipStream_t st) {
+    int rc = OK;
+    const size_t n = 4 * (size_t)d.n3() + d.n1() + d.n2() + 2;
+    double* b = (double*)workspace(WS_SUMS, sizeof(double) * n, &rc);
+    if (rc) return rc;
+    rc = zero_fill(b, sizeof(double) * n, st);
+    if (rc) return rc;
+    s->s3 = b;
+    s->s1 = b + 4 * d.n3();
+    s->s2 = s->s1 + d.n1();
+    if (mass) *mass = s->s2 + d.n2();
+    return OK;
+}
+
+static int check_modes(const PkGeometry* g) {
+    unsigned long long tot = 0;
+    for (auto v : g->h_n3) tot += v;
+    const long long N = g->d.N;
+    const long long own = (N % 2 == 1) ? 1 : 8;
+    const unsigned long long expect = (unsigned long long)((N * N * N - own) / 2 + own);
+    PYL_CHECK(tot == expect, E_MODES, "Not all modes counted: counted %llu, expected %llu", tot, expect);
+    return OK;
+}
+
+template <typename R>
+static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int mas_p, int flags, double* out,
+                         cplx<R>* keep, bool affine, const double* dmass, double mass, hipStream_t st) {
+    int rc = OK;
+    const int64_t P = half_len(dims);
+    const PkGeometry* g = pk_geometry(dims, axis, &rc);
+    if (rc) return rc;
+    rc = check_modes(g);
+    if (rc) return rc;
+    cplx<R>* spec = (cplx<R>*)workspace(WS_SPEC, sizeof(cplx<R>) * (size_t)dims * dims * P, &rc);
+    if (rc) return rc;
+    PkSums sums;
+    rc = pk_sums_ws(g->d, &sums, nullptr, st);
+    if (rc) return rc;
+    rc = fft_pass_z_r2c<R>(delta, spec, dims, P, affine, dmass, mass, st);
+    if (rc) return rc;
+    rc = fft_pass_cols<R>(spec, spec, dims, P, 1, false, (R)1, st);
+    if (rc) return rc;
+    rc = pk_x_pass_bin<R>(spec, keep, dims, P, axis, mas_p, flags, sums, st);
+    if (rc) return rc;
+    return pk_finalize(g, sums, BoxSize, out, st);
+}
+
+extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags,
+                      double* out, void* deltak_out, void* stream) {
+    int rc = check_fft(dims, prec);
+    if (rc) return rc;
+    PYL_CHECK(axis >= 0 && axis <= 2, E_ARG, "axis must be 0, 1 or 2");
+    PYL_CHECK(mas_p >= 0 && mas_p <= 4, E_ARG, "window exponent must be in [0, 4]");
+    PYL_CHECK(BoxSize > 0, E_ARG, "BoxSize must be positive");
+    PYL_CHECK(delta && out, E_ARG, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == PYL_F32)
+        return pk_from_delta<float>((const float*)delta, dims, BoxSize, axis, mas_p, flags, out,
+                                    (cplx<float>*)deltak_out, false, nullptr, 1.0, st);
+    return pk_from_delta<double>((const double*)delta, dims, BoxSize, axis, mas_p, flags, out,
+                                 (cplx<double>*)deltak_out, false, nullptr, 1.0, st);
+}
+
+extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
+                                int axis, int prec, int flags, double* out, void* stream) {
+    int rc = check_fft(dims, prec);
+    if (rc) return rc;
+    PYL_CHECK(axis >= 0 && axis <= 2, E_ARG, "axis must be 0, 1 or 2");
+    PYL_CHECK(scheme >= 1 && scheme <= 4, E_ARG, "unknown mass-assignment scheme %d", scheme);
+    PYL_CHECK(n > 0 && pos && out, E_ARG, "bad particle arguments");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t cells = (size_t)dims * dims * dims;
+    const bool f64 = prec == PYL_F64;
+    void* grid = workspace(WS_GRID, cells * (f64 ? 8 : 4), &rc);
+    if (rc) return rc;
+    double* dmass = nullptr;
+    if (W) {
+        dmass = (double*)workspace(WS_AUX3, 64, &rc);
+        if (rc) return rc;
+        rc = weight_sum(W, n, dmass, st);
+        if (rc) return rc;
+    }
+    rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | (f64 ? DEP_GRID_F64 : 0) | (flags & (DEP_ATOMIC | DEP_BINNED)),
+                 grid, st);
+    if (rc) return rc;
+    const int pkflags = flags & PYL_PK_ALL;
+    if (f64)
+        return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, true,
+                                     dmass, (double)n, st);
+    return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, true, dmass,
+                                (double)n, st);
+}

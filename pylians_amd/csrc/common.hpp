@@ -1,0 +1,65 @@
+// Shared helpers for the gfx950 kernels of libpylians_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+
+namespace pyl {
+
+// ---------------------------------------------------------------- errors
+// Thread-local last-error string surfaced through pyl_last_error().
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define PYL_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            ::pyl::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,              \
+                             hipGetErrorString(e_));                                   \
+            return (int)e_ + 1000;                                                     \
+        }                                                                              \
+    } while (0)
+
+#define PYL_CHECK(cond, code, ...)                                                     \
+    do {                                                                               \
+        if (!(cond)) {                                                                 \
+            ::pyl::set_error(__VA_ARGS__);                                             \
+            return (code);                                                             \
+        }                                                                              \
+    } while (0)
+
+enum Err { OK = 0, E_ARG = 1, E_UNSUPPORTED = 2, E_MODES = 3, E_ALLOC = 4, E_LAUNCH = 5 };
+
+// ---------------------------------------------------------------- complex
+template <typename R> struct cplx { R x, y; };
+
+template <typename R> __device__ __forceinline__ cplx<R> mk(R a, R b) { cplx<R> c; c.x = a; c.y = b; return c; }
+template <typename R> __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename R> __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+    return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename R> __device__ __forceinline__ cplx<R> scale(cplx<R> a, R s) { return mk(a.x * s, a.y * s); }
+template <typename R> __device__ __forceinline__ cplx<R> conjg(cplx<R> a) { return mk(a.x, -a.y); }
+// a * (-i)
+template <typename R> __device__ __forceinline__ cplx<R> mul_mi(cplx<R> a) { return mk(a.y, -a.x); }
+// a * (+i)
+template <typename R> __device__ __forceinline__ cplx<R> mul_pi(cplx<R> a) { return mk(-a.y, a.x); }
+
+// exact floor(sqrt(v)) for 0 <= v < 2^52
+__device__ __forceinline__ int isqrt_i(long long v) {
+    long long r = (long long)sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return (int)r;
+}
+
+// signed wavenumber of FFT index i on a grid of n points (Pk_library.pyx:315)
+__device__ __forceinline__ int kfreq(int i, int n) { return i > n / 2 ? i - n : i; }
+
+inline int64_t half_len(int n) { return (int64_t)(n / 2 + 1); }
+
+}  // namespace pyl

@@ -1,0 +1,25 @@
+// Mass assignment (particles -> grid) for gfx950.
+#pragma once
+#include "common.hpp"
+
+namespace pyl {
+
+enum DepositFlags {
+    DEP_GRID_F64 = 1,   // grid is float64 (CICW_d / NGPW_d, MAS_library.pyx:229-266, 338-358)
+    DEP_DUP2D = 2,      // 2-D Cython path: every weight deposited STENCIL times (MAS_library.pyx:137-139)
+    DEP_RENORM2D = 4,   // ... and the whole grid divided by STENCIL afterwards (MAS_library.pyx:84-110)
+    DEP_ZERO = 8,       // zero the grid before depositing (MA itself accumulates)
+    DEP_ATOMIC = 16,    // force the direct global-atomic kernel
+    DEP_BINNED = 32,    // force the cell-binned LDS kernel
+};
+
+// pos: (n, axes) float32 row-major device array; W nullable; grid (dims^axes) float32 or float64.
+// scheme: 1 NGP, 2 CIC, 3 TSC, 4 PCS.  The cell index and sub-cell fractions are computed in
+// float64 from the float32 positions; weights are float32 (float64 grid: float64).
+int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, double BoxSize, int scheme,
+            int flags, void* grid, hipStream_t st);
+
+// grid /= s (float32 or float64 grid of n cells), as numpy's `number2 /= 2.0` (MAS_library.pyx:89)
+int grid_divide(void* grid, int64_t n, bool f64, double s, hipStream_t st);
+
+}  // namespace pyl

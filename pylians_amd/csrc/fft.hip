@@ -1,0 +1,339 @@
+// 3-D real FFT passes for gfx950: rows r2c/c2r (pass Z) and column passes (Y, X).
+#include <math.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "fft.hpp"
+#include "fft_engine.hpp"
+
+namespace pyl {
+
+// ----------------------------------------------------------------------- kernels
+// Pass Z forward: NL rows of N = 2M reals, packed as M complex z[j] = x[2j] + i x[2j+1],
+// M-point FFT in LDS, then X[k] = E[k] + W_N^k O[k] for k in [0, M].
+template <typename R, int M, int NL, int T>
+__global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R>* __restrict__ out,
+                                                 long long nrows, long long P,
+                                                 const cplx<R>* __restrict__ twg, int affine,
+                                                 const double* __restrict__ dmass, double mass, double ncell) {
+    constexpr int N = 2 * M;
+    constexpr int ES = NL + 1;
+    __shared__ cplx<R> s[M * ES];
+    __shared__ cplx<R> tw[N];
+    const int tid = threadIdx.x;
+    // overdensity delta = rho/rhobar - 1, rhobar = mass/ncell (Pk_snapshot.py:84-88)
+    const R a = affine ? (R)(ncell / (dmass ? dmass[0] : mass)) : (R)1;
+    const R b = (R)-1;
+    const long long row0 = (long long)blockIdx.x * NL;
+    for (int i = tid; i < N; i += T) tw[i] = twg[i];
+    for (int idx = tid; idx < NL * M; idx += T) {
+        const int r = idx / M, j = idx % M;
+        const long long row = row0 + r;
+        cplx<R> v = mk<R>(0, 0);
+        if (row < nrows) {
+            const R* p = in + row * N + 2 * j;
+            v = mk(p[0], p[1]);
+            if (affine) {
+                v.x = v.x * a + b;
+                v.y = v.y * a + b;
+            }
+        }
+        s[j * ES + r] = v;
+    }
+    __syncthreads();
+    lds_fft<R, M, NL, ES, T, 2>(s, tw, tid);
+    for (int idx = tid; idx < NL * (M + 1); idx += T) {
+        const int r = idx / (M + 1), k = idx % (M + 1);
+        const long long row = row0 + r;
+        if (row >= nrows) continue;
+        const cplx<R> zk = s[(k % M) * ES + r];
+        const cplx<R> zm = conjg(s[((M - k) % M) * ES + r]);
+        const cplx<R> e = scale(zk + zm, (R)0.5);
+        const cplx<R> o = scale(mul_mi(zk - zm), (R)0.5);
+        out[row * P + k] = e + cmul(tw[k], o);
+    }
+}
+
+// Pass Z inverse: rows of M+1 complex -> N = 2M reals.
+// Z'[k] = (A + B) + i e^{+2 pi i k/N} (A - B), A = X[k], B = conj(X[M-k]); z = IFFT_M(Z');
+// x[2j] = Re z[j], x[2j+1] = Im z[j].  The inverse is computed as conj(FFT(conj(Z'))).
+// store: 0 = no output, 1 = x, 2 = x * aux.  Optional fused float64 reductions:
+// acc[0] += sum x^2, acc[1] += sum x*aux.
+template <typename R, int M, int NL, int T>
+__global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, R* __restrict__ out,
+                                                 long long nrows, long long P,
+                                                 const cplx<R>* __restrict__ twg, R sc, int store,
+                                                 const R* __restrict__ aux, double* __restrict__ acc) {
+    constexpr int N = 2 * M;
+    constexpr int ES = NL + 1;
+    __shared__ cplx<R> s[M * ES];
+    __shared__ cplx<R> tw[N];
+    __shared__ double red[2][T / 64];
+    const int tid = threadIdx.x;
+    const long long row0 = (long long)blockIdx.x * NL;
+    for (int i = tid; i < N; i += T) tw[i] = twg[i];
+    __syncthreads();
+    for (int idx = tid; idx < NL * M; idx += T) {
+        const int r = idx / M, k = idx % M;
+        const long long row = row0 + r;
+        cplx<R> v = mk<R>(0, 0);
+        if (row < nrows) {
+            const cplx<R> A = in[row * P + k];
+            const cplx<R> B = conjg(in[row * P + (M - k)]);
+            const cplx<R> e = A + B;
+            const cplx<R> o = cmul(A - B, conjg(tw[k]));
+            v = conjg(e + mul_pi(o));
+        }
+        s[k * ES + r] = v;
+    }
+    __syncthreads();
+    lds_fft<R, M, NL, ES, T, 2>(s, tw, tid);
+    double a0 = 0.0, a1 = 0.0;
+    for (int idx = tid; idx < NL * M; idx += T) {
+        const int r = idx / M, j = idx % M;
+        const long long row = row0 + r;
+        if (row >= nrows) continue;
+        const cplx<R> z = s[j * ES + r];
+        const R x0 = z.x * sc, x1 = -z.y * sc;
+        R q0 = (R)0, q1 = (R)0;
+        if (aux) {
+            const R* q = aux + row * N + 2 * j;
+            q0 = q[0];
+            q1 = q[1];
+        }
+        if (store) {
+            R* p = out + row * N + 2 * j;
+            p[0] = store == 2 ? x0 * q0 : x0;
+            p[1] = store == 2 ? x1 * q1 : x1;
+        }
+        if (acc) {
+            a0 += (double)x0 * x0 + (double)x1 * x1;
+            a1 += (double)x0 * q0 + (double)x1 * q1;
+        }
+    }
+    if (acc) {
+        for (int off = 32; off > 0; off >>= 1) {
+            a0 += __shfl_down(a0, off, 64);
+            a1 += __shfl_down(a1, off, 64);
+        }
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = a0;
+            red[1][tid >> 6] = a1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double s0 = 0.0, s1 = 0.0;
+            for (int w = 0; w < T / 64; ++w) {
+                s0 += red[0][w];
+                s1 += red[1][w];
+            }
+            atomicAdd(acc, s0);
+            if (aux) atomicAdd(acc + 1, s1);
+        }
+    }
+}
+
+// Column FFT over a half spectrum.  Column c (of ncols) starts at (c / CW) * CS + c % CW and
+// its N elements are EST apart.  Pass Y: CW = P, CS = N*P, EST = P.  Pass X: CW = ncols, EST = N*P.
+template <typename R, int N, int NL, int T, bool INV>
+__global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, long long ncols,
+                                             long long CW, long long CS, long long EST,
+                                             const cplx<R>* __restrict__ twg, R sc) {
+    static_assert(T % NL == 0, "T must be a multiple of NL");
+    __shared__ cplx<R> s[N * NL];
+    __shared__ cplx<R> tw[N];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N; i += T) tw[i] = twg[i];
+    const int l = tid % NL;
+    const long long c = (long long)blockIdx.x * NL + l;
+    const bool valid = c < ncols;
+    const long long base = valid ? (c / CW) * CS + (c % CW) : 0;
+    for (int j = tid / NL; j < N; j += T / NL) {
+        cplx<R> v = valid ? src[base + j * EST] : mk<R>(0, 0);
+        s[j * NL + l] = INV ? conjg(v) : v;
+    }
+    __syncthreads();
+    lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
+    if (valid) {
+        for (int j = tid / NL; j < N; j += T / NL) {
+            cplx<R> v = s[j * NL + l];
+            if (INV) v = conjg(v);
+            dst[base + j * EST] = scale(v, sc);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------- dispatch
+#define PYL_FFT_SIZES(X) \
+    X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
+    X(320) X(384) X(512) X(640) X(768) X(1024)
+
+bool fft_size_supported(int N) {
+    switch (N) {
+#define PYL_CASE(n) case n:
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        return true;
+        default:
+            return false;
+    }
+}
+
+template <typename R> const cplx<R>* twiddles(int N) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, void*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(dev, N);
+    auto it = cache.find(key);
+    if (it != cache.end()) return (const cplx<R>*)it->second;
+    std::vector<cplx<R>> h(N);
+    for (int t = 0; t < N; ++t) {
+        long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)N;
+        h[t].x = (R)cosl(ang);
+        h[t].y = (R)sinl(ang);
+    }
+    void* d = nullptr;
+    if (hipMalloc(&d, sizeof(cplx<R>) * N) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(cplx<R>) * N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    cache[key] = d;
+    return (const cplx<R>*)d;
+}
+template const cplx<float>* twiddles<float>(int);
+template const cplx<double>* twiddles<double>(int);
+
+template <typename R, int N>
+static int launch_r2c_rows(const R* in, cplx<R>* out, long long nrows, int64_t P, bool affine,
+                           const double* dmass, double mass, hipStream_t st) {
+    constexpr int M = N / 2;
+    constexpr int NL = Geo<R>::NL;
+    constexpr int T = threads_for(M, NL);
+    const cplx<R>* tw = twiddles<R>(N);
+    PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
+    dim3 grid((unsigned)((nrows + NL - 1) / NL));
+    hipLaunchKernelGGL((k_r2c_rows<R, M, NL, T>), grid, dim3(T), 0, st, in, out, nrows, (long long)P, tw,
+                       affine ? 1 : 0, dmass, mass, (double)N * N * N);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R, int N>
+static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P, R sc, int store,
+                           const R* aux, double* acc, hipStream_t st) {
+    constexpr int M = N / 2;
+    constexpr int NL = Geo<R>::NL;
+    constexpr int T = threads_for(M, NL);
+    const cplx<R>* tw = twiddles<R>(N);
+    PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
+    dim3 grid((unsigned)((nrows + NL - 1) / NL));
+    hipLaunchKernelGGL((k_c2r_rows<R, M, NL, T>), grid, dim3(T), 0, st, in, out, nrows, (long long)P, tw, sc,
+                       store, aux, acc);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R, int N>
+static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st) {
+    constexpr int NL = Geo<R>::NL;
+    constexpr int T = threads_for(N, NL);
+    const cplx<R>* tw = twiddles<R>(N);
+    PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
+    const long long ncols = (long long)N * P;
+    long long CW, CS, EST;
+    if (axis == 1) {
+        CW = P;
+        CS = (long long)N * P;
+        EST = P;
+    } else {
+        CW = ncols;
+        CS = 0;
+        EST = (long long)N * P;
+    }
+    dim3 grid((unsigned)((ncols + NL - 1) / NL));
+    if (inv)
+        hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, tw, sc);
+    else
+        hipLaunchKernelGGL((k_cols<R, N, NL, T, false>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, tw, sc);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R>
+int fft_pass_z_r2c(const R* in, cplx<R>* out, int N, int64_t P, bool affine, const double* dmass, double mass,
+                   hipStream_t st) {
+    const long long nrows = (long long)N * N;
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_r2c_rows<R, n>(in, out, nrows, P, affine, dmass, mass, st);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int fft_pass_cols(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, bool inverse, R sc,
+                  hipStream_t st) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_cols<R, n>(src, dst, P, axis, inverse, sc, st);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int fft_pass_z_c2r(const cplx<R>* in, R* out, int N, int64_t P, R sc, int store, const R* aux, double* acc,
+                   hipStream_t st) {
+    const long long nrows = (long long)N * N;
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_c2r_rows<R, n>(in, out, nrows, P, sc, store, aux, acc, st);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st) {
+    int rc = fft_pass_z_r2c<R>(in, out, N, P, false, nullptr, 1.0, st);
+    if (rc) return rc;
+    rc = fft_pass_cols<R>(out, out, N, P, 1, false, (R)1, st);
+    if (rc) return rc;
+    return fft_pass_cols<R>(out, out, N, P, 0, false, (R)1, st);
+}
+
+template <typename R>
+int fft3d_c2r(const cplx<R>* in, R* out, cplx<R>* ws, int N, int64_t P, R sc, hipStream_t st) {
+    int rc = fft_pass_cols<R>(in, ws, N, P, 0, true, (R)1, st);
+    if (rc) return rc;
+    rc = fft_pass_cols<R>(ws, ws, N, P, 1, true, (R)1, st);
+    if (rc) return rc;
+    return fft_pass_z_c2r<R>(ws, out, N, P, sc, 1, nullptr, nullptr, st);
+}
+
+#define PYL_INST(R)                                                                                        \
+    template int fft_pass_z_r2c<R>(const R*, cplx<R>*, int, int64_t, bool, const double*, double, hipStream_t);               \
+    template int fft_pass_cols<R>(const cplx<R>*, cplx<R>*, int, int64_t, int, bool, R, hipStream_t);         \
+    template int fft_pass_z_c2r<R>(const cplx<R>*, R*, int, int64_t, R, int, const R*, double*, hipStream_t); \
+    template int fft3d_r2c<R>(const R*, cplx<R>*, int, int64_t, hipStream_t);                                \
+    template int fft3d_c2r<R>(const cplx<R>*, R*, cplx<R>*, int, int64_t, R, hipStream_t);
+PYL_INST(float)
+PYL_INST(double)
+#undef PYL_INST
+
+}  // namespace pyl

@@ -1,0 +1,41 @@
+// Host-side interface of the 3-D real FFT (r2c / c2r) built from the LDS Stockham engine.
+//
+// Layout: real grid r[x][y][z] (N^3, z fastest); half spectrum c[x][y][kz], kz in [0, N/2],
+// row pitch P (complex elements; P = N/2+1 gives FFTW's / numpy's rfftn layout).
+// Forward = unnormalised (FFTW_FORWARD); inverse = unnormalised times `scale`.
+#pragma once
+#include "common.hpp"
+
+namespace pyl {
+
+bool fft_size_supported(int N);
+
+// twiddle table exp(-2 pi i t/N), t in [0, N), cached per (N, precision); device pointer
+template <typename R> const cplx<R>* twiddles(int N);
+
+// pass Z: rows r2c. out rows have P complex.  When `affine`, input values are mapped to the
+// overdensity v / rhobar - 1 with rhobar = mass / N^3, mass = dmass[0] (device) or `mass`.
+template <typename R>
+int fft_pass_z_r2c(const R* in, cplx<R>* out, int N, int64_t P, bool affine, const double* dmass, double mass,
+                   hipStream_t st);
+
+// column pass along y (axis 1) or x (axis 0) on a half spectrum, src -> dst (may alias)
+template <typename R>
+int fft_pass_cols(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, bool inverse, R scale,
+                  hipStream_t st);
+
+// pass Z inverse: rows c2r (unnormalised * scale).  store: 0 = nothing written, 1 = x,
+// 2 = x * aux.  When acc != nullptr the kernel also adds sum(x^2) into acc[0] and sum(x * aux)
+// into acc[1] (float64; aux == nullptr counts as zero).
+template <typename R>
+int fft_pass_z_c2r(const cplx<R>* in, R* out, int N, int64_t P, R scale, int store, const R* aux,
+                   double* acc, hipStream_t st);
+
+// full transforms
+template <typename R>
+int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st);
+// in is not modified; ws must hold N*N*P complex values (may equal nothing else)
+template <typename R>
+int fft3d_c2r(const cplx<R>* in, R* out, cplx<R>* ws, int N, int64_t P, R scale, hipStream_t st);
+
+}  // namespace pyl

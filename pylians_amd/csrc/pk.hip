@@ -1,0 +1,442 @@
+// Power spectrum on gfx950: bin geometry, the x-pass FFT fused with window deconvolution
+// and k-shell binning, and the normalisation of the outputs.
+//
+// Semantics follow Pk_library.pyx:266-425:
+//  * independent modes: kz in [0, N/2]; on the kz = 0 and kz = N/2 planes skip kx < 0, and where
+//    kx is 0 or N/2 also skip ky < 0 (Pk_library.pyx:322-330);
+//  * 3-D bin = floor(|k|) (integer k^2, exact isqrt), 1-D bin = |k_par| for |k| <= N/2,
+//    2-D bin = (kmax_par+1) * floor(sqrt(k_perp^2)) + |k_par| (Pk_library.pyx:334-373);
+//  * window factor prod_i (x_i/sin x_i)^p, x_i = pi k_i/N (Pk_library.pyx:75-87, 354-355);
+//  * multipole weights L2 = (3mu^2-1)/2, L4 = (35mu^4-30mu^2+3)/8, mu = k_par/|k| (:346-348, :376-380);
+//  * phase = atan2(Re, |delta|) (:361).
+// Per-mode arithmetic is in the working precision R; every shell sum is float64.
+#include <math.h>
+
+#include <map>
+#include <mutex>
+
+#include "fft.hpp"
+#include "fft_engine.hpp"
+#include "pk.hpp"
+
+namespace pyl {
+
+static int isqrt_host(long long v) {
+    long long r = (long long)sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return (int)r;
+}
+
+PkDims pk_dims(int N, int64_t P) {
+    PkDims d;
+    d.N = N;
+    d.middle = N / 2;
+    d.kmax_par = d.middle;
+    d.kmax_per = isqrt_host(2LL * d.middle * d.middle);
+    d.kmax = isqrt_host(3LL * d.middle * d.middle);
+    d.P = P;
+    return d;
+}
+
+size_t pk_out_len(const PkDims& d) { return (size_t)6 * d.kmax + 3 * (size_t)d.kmax_par + 4 * (size_t)d.n2(); }
+
+__device__ __forceinline__ bool indep_mode(int kx, int ky, int kz, int middle, bool even) {
+    if (kz == 0 || (kz == middle && even)) {
+        if (kx < 0) return false;
+        if ((kx == 0 || (kx == middle && even)) && ky < 0) return false;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ geometry
+__global__ void k_geom(int N, int axis, int kmax_par, int n1, int n3, unsigned long long* __restrict__ gn1,
+                       unsigned long long* __restrict__ gn2, unsigned long long* __restrict__ gn3,
+                       double* __restrict__ gk3) {
+    extern __shared__ double sm[];
+    double* k3 = sm;
+    unsigned* c3 = (unsigned*)(k3 + n3);
+    unsigned* c1 = c3 + n3;
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) {
+        k3[i] = 0.0;
+        c3[i] = 0u;
+    }
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) c1[i] = 0u;
+    __syncthreads();
+    const int middle = N / 2;
+    const bool even = (N % 2) == 0;
+    const long long Mz = middle + 1;
+    const long long total = (long long)N * N * Mz;
+    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int kzz = (int)(id % Mz);
+        const long long t = id / Mz;
+        const int kx = kfreq((int)(t / N), N), ky = kfreq((int)(t % N), N), kz = kzz;
+        if (!indep_mode(kx, ky, kz, middle, even)) continue;
+        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+        const int bin = isqrt_i(k2);
+        atomicAdd(&c3[bin], 1u);
+        atomicAdd(&k3[bin], sqrt((double)k2));
+        int kpar, kper;
+        if (axis == 0) {
+            kpar = kx;
+            kper = isqrt_i((long long)ky * ky + (long long)kz * kz);
+        } else if (axis == 1) {
+            kpar = ky;
+            kper = isqrt_i((long long)kx * kx + (long long)kz * kz);
+        } else {
+            kpar = kz;
+            kper = isqrt_i((long long)kx * kx + (long long)ky * ky);
+        }
+        kpar = kpar < 0 ? -kpar : kpar;
+        if (k2 <= (long long)middle * middle) atomicAdd(&c1[kpar], 1u);
+        atomicAdd(&gn2[(long long)(kmax_par + 1) * kper + kpar], 1ull);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) {
+        if (c3[i]) {
+            atomicAdd(&gn3[i], (unsigned long long)c3[i]);
+            atomicAdd(&gk3[i], k3[i]);
+        }
+    }
+    for (int i = threadIdx.x; i < n1; i += blockDim.x)
+        if (c1[i]) atomicAdd(&gn1[i], (unsigned long long)c1[i]);
+}
+
+const PkGeometry* pk_geometry(int N, int axis, int* rc) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, PkGeometry*> cache;
+    int dev = 0;
+    *rc = OK;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *rc = E_LAUNCH;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(dev, N, axis);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    PkGeometry* g = new PkGeometry();
+    g->d = pk_dims(N, half_len(N));
+    g->axis = axis;
+    const PkDims& d = g->d;
+    size_t b1 = sizeof(unsigned long long) * d.n1(), b2 = sizeof(unsigned long long) * d.n2(),
+           b3 = sizeof(unsigned long long) * d.n3(), bk = sizeof(double) * d.n3();
+    char* base = nullptr;
+    if (hipMalloc((void**)&base, b1 + b2 + b3 + bk) != hipSuccess) {
+        set_error("geometry allocation failed");
+        *rc = E_ALLOC;
+        delete g;
+        return nullptr;
+    }
+    g->k3sum = (double*)base;
+    g->n1 = (unsigned long long*)(base + bk);
+    g->n2 = (unsigned long long*)(base + bk + b1);
+    g->n3 = (unsigned long long*)(base + bk + b1 + b2);
+    hipError_t e = hipMemset(base, 0, b1 + b2 + b3 + bk);
+    size_t shm = sizeof(double) * d.n3() + sizeof(unsigned) * (d.n3() + d.n1());
+    hipLaunchKernelGGL(k_geom, dim3(1024), dim3(256), shm, 0, N, axis, d.kmax_par, d.n1(), d.n3(), g->n1, g->n2,
+                       g->n3, g->k3sum);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    g->h_n1.resize(d.n1());
+    g->h_n2.resize(d.n2());
+    g->h_n3.resize(d.n3());
+    g->h_k3sum.resize(d.n3());
+    if (e == hipSuccess) e = hipMemcpy(g->h_n1.data(), g->n1, b1, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(g->h_n2.data(), g->n2, b2, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(g->h_n3.data(), g->n3, b3, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(g->h_k3sum.data(), g->k3sum, bk, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_error("geometry kernel failed: %s", hipGetErrorString(e));
+        *rc = E_LAUNCH;
+        (void)hipFree(base);
+        delete g;
+        return nullptr;
+    }
+    cache[key] = g;
+    return g;
+}
+
+// ------------------------------------------------------------------ window tables
+// win[i] = (x/sin x)^p with x = pi kfreq(i)/N (1 at x = 0): Pk_library.pyx:86-87, 316-324
+static const double* window_table(int N, int p) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, double*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(dev, N, p);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> h(N);
+    for (int i = 0; i < N; ++i) {
+        int k = i > N / 2 ? i - N : i;
+        double x = M_PI / N * k;
+        h[i] = (x == 0.0) ? 1.0 : pow(x / sin(x), p);
+    }
+    double* dptr = nullptr;
+    if (hipMalloc((void**)&dptr, sizeof(double) * N) != hipSuccess) return nullptr;
+    if (hipMemcpy(dptr, h.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    cache[key] = dptr;
+    return dptr;
+}
+const double* pk_window_table(int N, int p) { return window_table(N, p); }
+
+// ------------------------------------------------------------------ x pass + binning
+constexpr int isqrt_c(long long v) {
+    long long r = 0;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return (int)r;
+}
+constexpr int hist_bytes(int N) {
+    return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * 4 * 8 + (N / 2 + 1) * 8;
+}
+// lines per block for the binning kernel: as many as fit beside the histograms
+template <typename R, int N> constexpr int nl_bin() {
+    int nl = Geo<R>::NL;
+    while (nl > 2 && N * nl * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + 8) + hist_bytes(N) > 150 * 1024)
+        nl /= 2;
+    return nl;
+}
+template <int N, int NL> constexpr int threads_bin() { return clamp_i(N * NL / 16, 64, 512); }
+
+template <typename R, int N, int NL, int T>
+__global__ __launch_bounds__(T) void k_x_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
+                                              long long P, int axis, int flags, int kmax_par, int n1, int n3,
+                                              const double* __restrict__ wing, const cplx<R>* __restrict__ twg,
+                                              double* __restrict__ g3, double* __restrict__ g1,
+                                              double* __restrict__ g2) {
+    constexpr int TPC = T / NL;  // threads per column
+    constexpr int E = (N + TPC - 1) / TPC;  // consecutive kx indices per thread in the epilogue
+    static_assert(T % NL == 0, "bad binning geometry");
+    __shared__ cplx<R> s[N * NL];
+    __shared__ cplx<R> tw[N];
+    __shared__ double win[N];
+    extern __shared__ double hs[];
+    double* h3 = hs;           // n3 * 4
+    double* h1 = hs + 4 * n3;  // n1
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N; i += T) {
+        tw[i] = twg[i];
+        win[i] = wing[i];
+    }
+    for (int i = tid; i < 4 * n3 + n1; i += T) hs[i] = 0.0;
+
+    const int middle = N / 2;
+    const bool even = (N % 2) == 0;
+    const long long ncols = (long long)N * P;
+    const long long EST = ncols;
+    const long long nblk = (ncols + NL - 1) / NL;
+    const int l = tid % NL;
+    const int jrow = tid / NL;
+    const bool do1 = flags & PK_1D, do2 = flags & PK_2D, doph = flags & PK_PHASE;
+
+    for (long long cb = blockIdx.x; cb < nblk; cb += gridDim.x) {
+        const long long c = cb * NL + l;
+        const bool valid = c < ncols;
+        __syncthreads();
+        for (int j = jrow; j < N; j += TPC) s[j * NL + l] = valid ? src[c + j * EST] : mk<R>(0, 0);
+        __syncthreads();
+        lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
+        if (!valid) continue;
+        const int kyy = (int)(c / P), kzz = (int)(c % P);
+        const int ky = kfreq(kyy, N), kz = kzz;
+        const int kyz2 = ky * ky + kz * kz;
+        const double cyz = win[kyy] * win[kzz];
+        const bool zplane = (kz == 0) || (kz == middle && even);
+        const int kper_fixed = isqrt_i(kyz2);
+        int cur3 = -1, cur1 = -1, cur2 = -1;
+        double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
+        const int j0 = jrow * E;
+#pragma unroll 4
+        for (int q = 0; q < E; ++q) {
+            const int j = j0 + q;
+            if (N % TPC != 0 && j >= N) break;
+            const int kx = kfreq(j, N);
+            cplx<R> v = s[j * NL + l];
+            const bool ind = !(zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0)));
+            if (!ind) {
+                if (keep) keep[c + j * EST] = v;
+                continue;
+            }
+            const R fac = (R)(win[j] * cyz);
+            v = scale(v, fac);
+            if (keep) keep[c + j * EST] = v;
+            const int k2 = kx * kx + kyz2;
+            const int bin = isqrt_i(k2);
+            int kpar = axis == 0 ? kx : (axis == 1 ? ky : kz);
+            const R mu2 = k2 == 0 ? (R)0 : (R)(kpar * kpar) / (R)k2;
+            const R d2 = v.x * v.x + v.y * v.y;
+            const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
+            const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
+            if (bin != cur3) {
+                if (cur3 >= 0) {
+                    atomicAdd(&h3[4 * cur3 + 0], a0);
+                    atomicAdd(&h3[4 * cur3 + 1], a2);
+                    atomicAdd(&h3[4 * cur3 + 2], a4);
+                    if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+                }
+                cur3 = bin;
+                a0 = a2 = a4 = ap = 0.0;
+            }
+            a0 += (double)d2;
+            a2 += (double)(d2 * L2);
+            a4 += (double)(d2 * L4);
+            if (doph) {
+                const R ph = atan2(v.x, sqrt(d2));
+                ap += (double)(ph * ph);
+            }
+            kpar = kpar < 0 ? -kpar : kpar;
+            if (do1 && k2 <= middle * middle) {
+                if (kpar != cur1) {
+                    if (cur1 >= 0) atomicAdd(&h1[cur1], b1);
+                    cur1 = kpar;
+                    b1 = 0.0;
+                }
+                b1 += (double)d2;
+            }
+            if (do2) {
+                const int kper = axis == 0 ? kper_fixed : isqrt_i(axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky);
+                const int i2 = (kmax_par + 1) * kper + kpar;
+                if (i2 != cur2) {
+                    if (cur2 >= 0) atomicAdd(&g2[cur2], b2);
+                    cur2 = i2;
+                    b2 = 0.0;
+                }
+                b2 += (double)d2;
+            }
+        }
+        if (cur3 >= 0) {
+            atomicAdd(&h3[4 * cur3 + 0], a0);
+            atomicAdd(&h3[4 * cur3 + 1], a2);
+            atomicAdd(&h3[4 * cur3 + 2], a4);
+            if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+        }
+        if (cur1 >= 0) atomicAdd(&h1[cur1], b1);
+        if (cur2 >= 0) atomicAdd(&g2[cur2], b2);
+    }
+    __syncthreads();
+    for (int i = tid; i < 4 * n3; i += T)
+        if (hs[i] != 0.0) atomicAdd(&g3[i], hs[i]);
+    for (int i = tid; i < n1; i += T)
+        if (h1[i] != 0.0) atomicAdd(&g1[i], h1[i]);
+}
+
+static int num_cus() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
+template <typename R, int N>
+static int launch_x_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
+                        const PkSums& sums, hipStream_t st) {
+    constexpr int NL = nl_bin<R, N>();
+    constexpr int T = threads_bin<N, NL>();
+    const PkDims d = pk_dims(N, P);
+    const cplx<R>* tw = twiddles<R>(N);
+    const double* win = window_table(N, p);
+    PYL_CHECK(tw && win, E_ALLOC, "table allocation failed");
+    const long long ncols = (long long)N * P;
+    const long long nblk = (ncols + NL - 1) / NL;
+    long long grid = (long long)num_cus() * 4;
+    if (grid > nblk) grid = nblk;
+    size_t shm = sizeof(double) * (4 * (size_t)d.n3() + d.n1());
+    hipLaunchKernelGGL((k_x_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P, axis,
+                       flags, d.kmax_par, d.n1(), d.n3(), win, tw, sums.s3, sums.s1, sums.s2);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+#define PYL_PK_SIZES(X) \
+    X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) \
+    X(320) X(384) X(512) X(640) X(768) X(1024)
+
+template <typename R>
+int pk_x_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
+                  const PkSums& sums, hipStream_t st) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_x_bin<R, n>(spec, keep, P, axis, mas_p, flags, sums, st);
+        PYL_PK_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("grid size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+template int pk_x_pass_bin<float>(const cplx<float>*, cplx<float>*, int, int64_t, int, int, int, const PkSums&,
+                                  hipStream_t);
+template int pk_x_pass_bin<double>(const cplx<double>*, cplx<double>*, int, int64_t, int, int, int, const PkSums&,
+                                   hipStream_t);
+
+// ------------------------------------------------------------------ finalize
+__global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxSize,
+                              const unsigned long long* __restrict__ n1c, const unsigned long long* __restrict__ n2c,
+                              const unsigned long long* __restrict__ n3c, const double* __restrict__ k3s,
+                              const double* __restrict__ s3, const double* __restrict__ s1,
+                              const double* __restrict__ s2, double* __restrict__ out) {
+    // evaluate as written (no fused multiply-add), as the reference's Python arithmetic does:
+    // the last 1-D bin needs kN^2 - k^2 == 0 exactly
+#pragma clang fp contract(off)
+    const double kF = 2.0 * M_PI / BoxSize;
+    const int middle = N / 2;
+    const double kN = middle * kF;
+    const double unit = (BoxSize / ((double)N * N)) * (BoxSize / ((double)N * N)) * (BoxSize / ((double)N * N));
+    double* k3D = out;
+    double* P0 = k3D + kmax;
+    double* P2 = P0 + kmax;
+    double* P4 = P2 + kmax;
+    double* ph = P4 + kmax;
+    double* N3 = ph + kmax;
+    double* k1D = N3 + kmax;
+    double* P1 = k1D + kmax_par;
+    double* N1 = P1 + kmax_par;
+    double* kpar = N1 + kmax_par;
+    double* kper = kpar + n2;
+    double* P2D = kper + n2;
+    double* N2 = P2D + n2;
+    for (int i = threadIdx.x; i < kmax; i += blockDim.x) {
+        const double n = (double)n3c[i + 1];
+        k3D[i] = (k3s[i + 1] / n) * kF;
+        P0[i] = (s3[4 * (i + 1) + 0] / n) * unit;
+        P2[i] = (s3[4 * (i + 1) + 1] * 5.0 / n) * unit;
+        P4[i] = (s3[4 * (i + 1) + 2] * 9.0 / n) * unit;
+        ph[i] = (s3[4 * (i + 1) + 3] / n) * unit;
+        N3[i] = n;
+    }
+    for (int i = threadIdx.x; i < kmax_par; i += blockDim.x) {
+        const double n = (double)n1c[i + 1];
+        const double k = ((double)(i + 1) * n / n) * kF;
+        const double kmaxper = sqrt(kN * kN - k * k);
+        k1D[i] = k;
+        P1[i] = s1[i + 1] * unit * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
+        N1[i] = n;
+    }
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const int kp = i % (kmax_par + 1), kq = i / (kmax_par + 1);
+        const double n = (double)n2c[i];
+        kpar[i] = 0.5 * (kp + kp + 1) * kF;
+        kper[i] = 0.5 * (kq + kq + 1) * kF;
+        P2D[i] = s2[i] * unit / n;
+        N2[i] = n;
+    }
+}
+
+int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, double* out, hipStream_t st) {
+    const PkDims& d = g->d;
+    hipLaunchKernelGGL(k_pk_finalize, dim3(1), dim3(1024), 0, st, d.N, d.kmax, d.kmax_par, d.n2(), BoxSize, g->n1,
+                       g->n2, g->n3, g->k3sum, sums.s3, sums.s1, sums.s2, out);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+}  // namespace pyl

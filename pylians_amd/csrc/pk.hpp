@@ -1,0 +1,56 @@
+// Power-spectrum estimator pieces (Pk_library.pyx:266-425 semantics) for gfx950.
+#pragma once
+#include "common.hpp"
+
+namespace pyl {
+
+struct PkDims {
+    int N, middle, kmax_par, kmax_per, kmax;
+    int64_t P;                 // half-spectrum row pitch (complex elements)
+    int n1() const { return kmax_par + 1; }
+    int n2() const { return (kmax_par + 1) * (kmax_per + 1); }
+    int n3() const { return kmax + 1; }
+};
+PkDims pk_dims(int N, int64_t P);
+
+// Data-independent bin geometry of the independent modes (Pk_library.pyx:322-330):
+// integer mode counts per 1-D / 2-D / 3-D bin and the float64 sum of |k| per 3-D bin
+// (kF units).  Computed once per (device, N, axis) on the GPU and cached.
+struct PkGeometry {
+    PkDims d;
+    int axis;
+    // device arrays
+    unsigned long long* n1;    // n1()
+    unsigned long long* n2;    // n2()
+    unsigned long long* n3;    // n3()
+    double* k3sum;             // n3()
+    // host copies
+    std::vector<unsigned long long> h_n1, h_n2, h_n3;
+    std::vector<double> h_k3sum;
+};
+const PkGeometry* pk_geometry(int N, int axis, int* rc);
+
+// float64 per-bin sums of the data-dependent quantities, device arrays owned by caller
+struct PkSums {
+    double* s3;   // n3 * 4: sum d2, sum d2 L2(mu), sum d2 L4(mu), sum phase^2
+    double* s1;   // n1: sum d2 over modes with |k| <= N/2, binned by |k_par|
+    double* s2;   // n2: sum d2 over 2-D bins (kmax_par+1)*k_per + k_par
+};
+
+enum PkFlags { PK_1D = 1, PK_2D = 2, PK_PHASE = 4 };
+
+// Last FFT pass (along x) of a half spectrum that already went through passes Z and Y,
+// fused with the window deconvolution and the binning of the independent modes.
+// keep (nullable) receives the window-corrected spectrum (Pk_library.pyx:354-355, 423).
+template <typename R>
+int pk_x_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
+                  const PkSums& sums, hipStream_t st);
+
+// Normalisation of Pk_library.pyx:387-421 on the device.  out layout (float64):
+//   [k3D (kmax) | P0 (kmax) | P2 (kmax) | P4 (kmax) | Pkphase (kmax) | Nmodes3D (kmax) |
+//    k1D (kmax_par) | Pk1D (kmax_par) | Nmodes1D (kmax_par) |
+//    kpar (n2) | kper (n2) | Pk2D (n2) | Nmodes2D (n2)]
+size_t pk_out_len(const PkDims& d);
+int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, double* out, hipStream_t st);
+
+}  // namespace pyl

@@ -1,0 +1,34 @@
+"""GPU parity of Pk_library.Bk against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import bk as Bo
+from oracle import cpu as C
+from oracle import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def PKL(gpu):
+    from pylians_amd import Pk_library
+    return Pk_library
+
+
+@pytest.mark.parametrize("N,prec", [(32, "f32"), (48, "f32"), (32, "f64")])
+def test_bk_parity(PKL, N, prec):
+    L = 400.0
+    pos = synth.zeldovich(N, L, seed=N, rms_disp=6.0)
+    rho = C.deposit_f64(pos, N, L, "CIC")
+    delta = rho / rho.mean() - 1.0
+    delta = delta.astype(np.float32 if prec == "f32" else np.float64)
+    kF = 2 * np.pi / L
+    k1, k2 = 4.3 * kF, 7.1 * kF
+    theta = np.linspace(0, np.pi, 6)
+    g = PKL.Bk(delta, L, k1, k2, theta, MAS="CIC", threads=2)
+    o = Bo.Bk(delta.astype(np.float64), L, k1, k2, theta, MAS="CIC")
+    tol = 1e-5 if prec == "f32" else 1e-10
+    assert np.allclose(g.k, o.k, rtol=1e-14)
+    assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= tol
+    assert np.max(np.abs(g.B - o.B) / np.abs(o.B)) <= tol, (g.B, o.B)
+    assert np.max(np.abs(g.Q - o.Q) / np.abs(o.Q)) <= tol

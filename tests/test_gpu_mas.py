@@ -1,0 +1,122 @@
+"""GPU parity of the mass assignment (drop-in MAS_library) against the FP64 oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu as C
+from oracle import mas as M
+from oracle import synth
+from tests._cases import SCHEMES, places_ok, ref_mas_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def MASL(gpu):
+    from pylians_amd import MAS_library
+    return MAS_library
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_reference_unit_cases(MASL, scheme, weighted):
+    """Test/test_MAS.py:10-238 through MA, the Cython-style and the MAS_c-style entry points."""
+    W = np.full(1000, 3.0, np.float32) if weighted else None
+    norm = 1000 * (3.0 if weighted else 1.0)
+    places = 20 if scheme == "NGP" else 8
+    for axes in (3, 2):
+        pos = ref_mas_case(axes)
+        d = np.zeros((64,) * axes, np.float32)
+        MASL.MA(pos, d, 1.0, scheme, W=W)
+        assert places_ok(np.sum(d, dtype=np.float64) / norm, places)
+        d = np.zeros((64,) * axes, dtype=MASL.FLOAT_type())
+        suffix = ("W" if weighted else "") + ("c3D" if axes == 3 else "c2D")
+        fn = getattr(MASL, scheme + suffix)
+        fn(pos, d, W, 1.0, 2) if weighted else fn(pos, d, 1.0, 2)
+        assert places_ok(np.sum(d, dtype=np.float64) / norm, places)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("axes", [3, 2])
+def test_per_cell_parity(MASL, scheme, weighted, axes):
+    rng = np.random.default_rng(7)
+    n, dims, L = 200000, 64, 1000.0
+    pos = rng.random((n, axes), dtype=np.float32) * np.float32(L)
+    W = (rng.random(n, dtype=np.float32) + np.float32(0.25)) if weighted else None
+    g = np.zeros((dims,) * axes, np.float32)
+    MASL.MA(pos, g, L, scheme, W=W)
+    o = M.MA(pos, np.zeros((dims,) * axes), L, scheme, W)
+    err = np.abs(g - o)
+    # per cell: relative on rho (float32 weights and accumulation), no cell exempt
+    assert np.all(err <= 1e-5 * np.abs(o) + 1e-12), (err / np.maximum(np.abs(o), 1e-30)).max()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_zeldovich_parity_and_float64_grid(MASL, scheme):
+    pos = synth.zeldovich(48, 300.0, seed=5, rms_disp=3.0, sigma_v=200.0)
+    dims = 48
+    g = np.zeros((dims,) * 3, np.float32)
+    MASL.MA(pos, g, 300.0, scheme)
+    o = C.deposit_f64(pos, dims, 300.0, scheme)
+    assert np.all(np.abs(g - o) <= 1e-5 * np.abs(o) + 1e-12)
+    if scheme in ("CIC", "NGP"):
+        W = np.random.default_rng(1).random(len(pos), dtype=np.float32)
+        gd = np.zeros((dims,) * 3, np.float64)
+        getattr(MASL, scheme + "W_d")(pos, gd, 300.0, W)
+        od = C.deposit_f64(pos, dims, 300.0, scheme, W)
+        assert np.allclose(gd, od, rtol=1e-12, atol=1e-12)
+
+
+def test_accumulates_and_device_inplace(MASL):
+    rng = np.random.default_rng(9)
+    pos = rng.random((5000, 3), dtype=np.float32) * np.float32(10.0)
+    start = rng.random((16, 16, 16)).astype(np.float32)
+    g = start.copy()
+    MASL.MA(pos, g, 10.0, "TSC")
+    o = M.MA(pos, start.astype(np.float64), 10.0, "TSC")
+    assert np.allclose(g, o, rtol=1e-5, atol=0)
+    t = torch.from_numpy(start.copy()).cuda()
+    MASL.MA(torch.from_numpy(pos).cuda(), t, 10.0, "TSC")
+    assert np.allclose(t.cpu().numpy(), o, rtol=1e-5, atol=0)
+
+
+def test_2d_renormalisation_semantics(MASL):
+    rng = np.random.default_rng(12)
+    pos = rng.random((3000, 2), dtype=np.float32)
+    prev = rng.random((32, 32)).astype(np.float32)
+    for ren in (True, False):
+        g = prev.copy()
+        MASL.MA(pos, g, 1.0, "PCS", renormalize_2D=ren)
+        o = M.MA(pos, prev.astype(np.float64), 1.0, "PCS", renormalize_2D=ren)
+        assert np.allclose(g, o, rtol=2e-6, atol=1e-7)
+
+
+def test_errors(MASL):
+    with pytest.raises(SystemExit):
+        MASL.MA(np.zeros((3, 3), np.float32), np.zeros((4, 4), np.float32), 1.0)
+    with pytest.raises(SystemExit):
+        MASL.MA(np.zeros((3, 3), np.float32), np.zeros((4, 4, 4), np.float32), 1.0, "QQQ")
+    with pytest.raises(ValueError):
+        MASL.MA(np.zeros((3, 3), np.float32), np.zeros((4, 4, 4), np.float64), 1.0)
+
+
+def test_mas_c_host_entry_points(gpu):
+    """pyl_mas_CIC etc. take host pointers exactly like MAS_c.h:3-10."""
+    from pylians_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(21)
+    pos = rng.random((20000, 3), dtype=np.float32) * np.float32(50.0)
+    W = rng.random(20000, dtype=np.float32)
+    for s in SCHEMES:
+        g = np.zeros((32,) * 3, np.float32)
+        rc = getattr(L, "pyl_mas_" + s)(pos.ctypes.data_as(ctypes.c_void_p), g.ctypes.data_as(ctypes.c_void_p),
+                                       W.ctypes.data_as(ctypes.c_void_p), len(pos), 32, 3, 50.0, 4)
+        assert rc == 0
+        o = C.deposit_f64(pos, 32, 50.0, s, W)
+        assert np.allclose(g, o, rtol=1e-5, atol=1e-9)
+        if C.ref_available():
+            r = C.ref_mas_c(pos, np.zeros((32,) * 3, np.float32), 50.0, s, W, threads=4)
+            assert np.allclose(g, r, rtol=1e-4, atol=1e-5)

@@ -1,0 +1,81 @@
+"""GPU parity of Pk_library.Pk (1-D, 2-D, 3-D, multipoles, phase, Nmodes) against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu as C
+from oracle import synth
+from tests._gpuref import assert_pk_close, oracle_pk
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def PKL(gpu):
+    from pylians_amd import Pk_library
+    return Pk_library
+
+
+def density_field(N, L, seed, MAS="CIC"):
+    pos = synth.zeldovich(N, L, seed=seed, rms_disp=0.02 * L, sigma_v=0.0)
+    rho = C.deposit_f64(pos, N, L, MAS)
+    return (rho / rho.mean() - 1.0).astype(np.float32)
+
+
+@pytest.mark.parametrize("N", [16, 32, 64, 96, 128])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_pk_float32_parity(PKL, N, axis):
+    L = 500.0
+    delta = density_field(N, L, seed=N + axis)
+    g = PKL.Pk(delta, L, axis=axis, MAS="CIC", threads=4)
+    o = oracle_pk(delta, L, axis, "CIC")
+    errs = assert_pk_close(g, o, rtol=1e-5)
+    print("N=%d axis=%d max rel err vs FP64 oracle: %s" % (N, axis, errs))
+
+
+@pytest.mark.parametrize("MAS", ["NGP", "TSC", "PCS", "None"])
+def test_pk_windows(PKL, MAS):
+    N, L = 64, 250.0
+    delta = density_field(N, L, seed=3, MAS=MAS if MAS != "None" else "CIC")
+    g = PKL.Pk(delta, L, axis=2, MAS=MAS)
+    o = oracle_pk(delta, L, 2, MAS)
+    assert_pk_close(g, o, rtol=1e-5)
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_pk_float64_path(PKL, N):
+    L = 1000.0
+    delta = density_field(N, L, seed=N).astype(np.float64)
+    g = PKL.Pk(delta, L, axis=0, MAS="TSC")
+    o = oracle_pk(delta, L, 0, "TSC")
+    assert_pk_close(g, o, rtol=1e-10)
+
+
+def test_keep_deltak(PKL):
+    N, L = 32, 100.0
+    delta = density_field(N, L, seed=1)
+    g = PKL.Pk(delta, L, axis=1, MAS="PCS", keep_deltak=True)
+    o = oracle_pk(delta, L, 1, "PCS", keep=True)
+    assert g.delta_k.dtype == np.complex64
+    assert np.abs(g.delta_k - o.delta_k).max() <= 1e-5 * np.abs(o.delta_k).max()
+
+
+def test_device_input_and_odd_free_sizes(PKL):
+    N, L = 48, 300.0
+    delta = density_field(N, L, seed=4)
+    g = PKL.Pk(torch.from_numpy(delta).cuda(), L, axis=0, MAS="CIC")
+    o = oracle_pk(delta, L, 0, "CIC")
+    assert_pk_close(g, o, rtol=1e-5)
+
+
+def test_plane_wave_closed_form(PKL):
+    N, L, n = 32, 200.0, 4
+    x = np.arange(N)
+    delta = np.ascontiguousarray(np.broadcast_to(np.cos(2 * np.pi * n * x / N)[None, :, None],
+                                                 (N, N, N)).astype(np.float32))
+    p = PKL.Pk(delta, L, axis=1, MAS="None")
+    b = n - 1
+    assert p.Pk[b, 0] == pytest.approx(L ** 3 / (4 * p.Nmodes3D[b]), rel=1e-5)
+    assert p.Pk[b, 1] == pytest.approx(5 * p.Pk[b, 0], rel=1e-5)
+    assert p.Pk[b, 2] == pytest.approx(9 * p.Pk[b, 0], rel=1e-5)
+    assert int(p.Nmodes3D.sum()) + 1 == (N ** 3 - 8) // 2 + 8

@@ -1,0 +1,54 @@
+"""Pins the bispectrum oracle (oracle/bk.py) by brute-force enumeration of closed triangles
+k1 + k2 + k3 = 0 (mod N) over the full-space shells, which equals the real-space triple sum of
+the inverse-transformed shell fields by the convolution theorem."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import bk as Bo
+from oracle import pk as P
+
+
+def test_brute_force_triangles():
+    N, L = 12, 2 * np.pi          # kF = 1: shells in integer units
+    rng = np.random.default_rng(0)
+    delta = rng.standard_normal((N, N, N))
+    k1, k2 = 2.0, 3.0
+    theta = np.array([0.3, 1.2, 2.5])
+    b = Bo.Bk(delta, L, k1, k2, theta, MAS="CIC")
+    full = np.fft.fftn(delta)
+    rep = lambda i: i - N if i > N // 2 else i
+    win = lambda i: P.MAS_correction(np.pi * rep(i) / N, 2)
+    allk = list(itertools.product(range(N), repeat=3))
+    kabs = {k: np.sqrt(sum(rep(c) ** 2 for c in k)) for k in allk}
+    W = {k: win(k[0]) * win(k[1]) * win(k[2]) for k in allk}
+    k_all, kmin, kmax = Bo.shell_bounds(L, N, k1, k2, theta)
+    shells = [[k for k in allk if kmin[i] <= kabs[k] < kmax[i]] for i in range(len(k_all))]
+    unitP = (L / N ** 2) ** 3
+    unitB = (L ** 2 / N ** 3) ** 3
+    for i, S in enumerate(shells):
+        Pi = np.mean([abs(full[k] * W[k]) ** 2 for k in S]) * unitP
+        assert b.Pk[i] == pytest.approx(Pi, rel=1e-10)
+    S1, S2 = shells[0], shells[1]
+    for j in range(len(theta)):
+        S3 = set(shells[j + 2])
+        tot, cnt = 0.0, 0
+        for a in S1:
+            for c in S2:
+                t = tuple((-a[d] - c[d]) % N for d in range(3))
+                if t in S3:
+                    tot += (full[a] * W[a] * full[c] * W[c] * full[t] * W[t]).real
+                    cnt += 1
+        assert cnt > 0
+        assert b.B[j] == pytest.approx(tot / cnt * unitB, rel=1e-9)
+        Pj = b.Pk
+        assert b.Q[j] == pytest.approx(b.B[j] / (Pj[0] * Pj[1] + Pj[0] * Pj[j + 2] + Pj[1] * Pj[j + 2]))
+    assert np.allclose(b.k[2:], np.sqrt((k2 * np.sin(theta)) ** 2 + (k2 * np.cos(theta) + k1) ** 2))
+
+
+def test_empty_shell_raises():
+    N = 8
+    delta = np.random.default_rng(1).standard_normal((N, N, N))
+    with pytest.raises(ZeroDivisionError):
+        Bo.Bk(delta, 2 * np.pi, 100.0, 2.0, np.array([0.5]))
