@@ -1,0 +1,187 @@
+#!/usr/bin/env python
+"""Benchmark of the density-assignment -> P(k) path on one MI355X.
+
+Workload (default): 512^3 particles from a Zel'dovich-displaced 512^3 lattice (Gaussian field
+P(k) = A k^-2 exp(-(k/0.5)^2), rms displacement 5 Mpc/h, L = 1000 Mpc/h) shifted to redshift space
+along axis 0 by v ~ N(0, 300 km/s); CIC onto a 512^3 grid, delta = rho/rhobar - 1, r2c, CIC
+deconvolution and P0/P2/P4 (+ 1-D, 2-D, phase) binning, all in one fused library call
+(pyl_pk_particles).  A step is one pass over the particle set with positions resident in HBM.
+
+Prints ONE JSON line with the metric, the float64 GPU path's time (speedup), the roofline of the
+dominant kernel (HIP events recorded by the library on the stream it launches on) and the
+CPU baseline (the reference's own MAS_c.c + pocketfft + the restated serial mode loop) timed on
+this host, plus the error of the float32-compensated result against the FP64 oracle.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "Mparticles/s CIC+Pk end-to-end at 512^3; FP32 vs FP64 speedup at 1e-5 tol"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n_p, N):
+    """Per-stage compulsory bytes (input read once, materialised output written once)."""
+    Ng = N ** 3
+    Nh = N * N * (N // 2 + 1)
+    return {
+        "deposit_atomic": 12 * n_p + 4 * Ng,     # read positions, write the float32 grid
+        "deposit_binned": 12 * n_p + 4 * Ng,
+        "grid_zero": 4 * Ng,
+        "fft_z_r2c": 4 * Ng + 8 * Nh,            # read the real grid, write the half spectrum
+        "fft_y": 16 * Nh,                         # read + write the half spectrum
+        "fft_x_bin": 8 * Nh,                      # read the half spectrum (histograms negligible)
+        "e2e": (12 * n_p + 4 * Ng) + (4 * Ng + 8 * Nh) + 8 * Nh,
+    }
+
+
+def cpu_leg(pos_host, N, L, axis, gpu_pk, threads):
+    """Reference CPU path timed on this host + FP64 oracle error of the GPU result (checker only)."""
+    from oracle import cpu as C
+
+    out = {}
+    if C.ref_available():
+        res, t = C.cpu_reference_pipeline(pos_host, N, L, axis, threads)
+        out["cpu_baseline"] = {
+            "value": round(len(pos_host) / t["total"] / 1e6, 3),
+            "unit": "Mparticles/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": ("full workload once: reference MAS_c.c CIC (OpenMP, %d threads) %.2fs + float32 "
+                       "normalise + scipy.fft r2c (%d workers; FFTW absent) %.2fs + serial C restatement of "
+                       "the Pk mode loop %.2fs" % (threads, t["deposit"], threads, t["normalise_fft"], t["modes"])),
+        }
+    o, fl = C.fp64_oracle_pipeline(pos_host, N, L, axis, workers=threads, with_floor=True)
+    P0 = o.Pk[:, 0]
+    err = {
+        "P0": float(np.max(np.abs(gpu_pk.Pk[:, 0] - P0) / np.abs(P0))),
+        "P2_over_P0": float(np.max(np.abs(gpu_pk.Pk[:, 1] - o.Pk[:, 1]) / np.abs(P0))),
+        "P4_over_P0": float(np.max(np.abs(gpu_pk.Pk[:, 2] - o.Pk[:, 2]) / np.abs(P0))),
+        "Nmodes3D_exact": bool(np.array_equal(gpu_pk.Nmodes3D, o.Nmodes3D)),
+        "k3D": float(np.max(np.abs(gpu_pk.k3D - o.k3D) / o.k3D)),
+        "bins_P0_within_1e-5": float(np.mean(np.abs(gpu_pk.Pk[:, 0] - P0) / np.abs(P0) <= 1e-5)),
+        "bins_P2P4_within_1e-5_of_P0": float(np.mean(np.abs(gpu_pk.Pk[:, 1:] - o.Pk[:, 1:]) / np.abs(P0)[:, None] <= 1e-5)),
+        "f32_grid_floor_P0": float(np.max(np.abs(fl.Pk[:, 0] - P0) / np.abs(P0))),
+        "f32_grid_floor_P4_over_P0": float(np.max(np.abs(fl.Pk[:, 2] - o.Pk[:, 2]) / np.abs(P0))),
+    }
+    if "cpu_baseline" in out:
+        err["cpu_reference_f32_P0"] = float(np.max(np.abs(res.Pk[:, 0] - P0) / np.abs(P0)))
+    out["err_vs_fp64_oracle"] = err
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-side", type=int, default=512, help="particles per side of the lattice")
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--mas", default="CIC")
+    ap.add_argument("--deposit", default=None, choices=[None, "atomic", "binned"])
+    ap.add_argument("--fp64-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 oracle check")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pylians_amd import _lib, synth
+    from pylians_amd.Pk_library import pk_layout, unpack_pk
+    from pylians_amd.pipeline import run_device
+
+    _lib.require_gpu()
+    n_side, N, L, axis = args.n_side, args.grid, 1000.0, 0
+    scheme = _lib.SCHEMES[args.mas]
+    t0 = time.time()
+    pos = synth.zeldovich_device(n_side, L, seed=2, rms_disp=5.0, kc=0.5, sigma_v=300.0, axis=axis)
+    torch.cuda.synchronize()
+    n_p = int(pos.shape[0])
+    print("# generated %d particles in %.1fs" % (n_p, time.time() - t0), file=sys.stderr, flush=True)
+    _, _, _, nout = pk_layout(N)
+    out = torch.empty(nout, dtype=torch.float64, device="cuda")
+    flags = _lib.PK_ALL | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED}[args.deposit]
+
+    def step(prec):
+        run_device(pos, None, out, L, N, scheme, axis, prec, flags)
+
+    # ---- float32-compensated path: warmup, then the timed region with stage events
+    for _ in range(args.warmup):
+        step(_lib.F32)
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        step(_lib.F32)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    stages = _lib.profile_read()
+    _lib.profile(False)
+    ms = ev0.elapsed_time(ev1) / args.steps
+    res32 = out.cpu().numpy().copy()
+
+    class R:
+        pass
+
+    gpu_pk = R()
+    unpack_pk(gpu_pk, res32, N)
+    print("# fp32c: %.3f ms/step (wall %.3f ms/step)" % (ms, wall * 1e3 / args.steps), file=sys.stderr, flush=True)
+
+    # ---- plain float64 GPU path (same particles, float64 grid, FFT and sums)
+    step(_lib.F64)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.fp64_steps):
+        step(_lib.F64)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms64 = ev0.elapsed_time(ev1) / args.fp64_steps
+    res64 = R()
+    unpack_pk(res64, out.cpu().numpy(), N)
+    print("# fp64: %.3f ms/step" % ms64, file=sys.stderr, flush=True)
+
+    # ---- roofline of the dominant kernel
+    ab = algorithmic_bytes(n_p, N)
+    per = {k: (v[1] / v[0]) for k, v in stages.items() if v[0] > 0}
+    dom = max((k for k in per if k in ab), key=lambda k: per[k])
+    achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": ab[dom], "avg_ms": round(per[dom], 4)}
+    stage_ms = {k: round(v, 4) for k, v in per.items()}
+
+    line = {
+        "metric": METRIC, "value": round(n_p / (ms * 1e-3) / 1e6, 1), "unit": "Mparticles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (Zel'dovich lattice + RSD, seeded, generated on the GPU)",
+        "config": {"workload": "zeldovich%d_rsd_%s_pk_grid%d" % (n_side, args.mas.lower(), N), "n_particles": n_p,
+                   "grid": N, "BoxSize": L, "MAS": args.mas, "axis": axis, "outputs": "P0/P2/P4, 1-D, 2-D, phase",
+                   "deposit": args.deposit or "auto"},
+        "fp64_ms_per_step": round(ms64, 4), "speedup_fp32c_vs_fp64": round(ms64 / ms, 3),
+        "grids_per_s": round(1e3 / ms, 3),
+        "e2e_hbm_frac": round(ab["e2e"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": stage_ms, "roofline": roofline,
+        "fp64_vs_fp32c_P0_maxrel": float(np.max(np.abs(res64.Pk[:, 0] - gpu_pk.Pk[:, 0]) / np.abs(res64.Pk[:, 0]))),
+    }
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        pos_host = pos.cpu().numpy()
+        t1 = time.time()
+        line.update(cpu_leg(pos_host, N, L, axis, gpu_pk, threads))
+        print("# cpu leg %.1fs" % (time.time() - t1), file=sys.stderr, flush=True)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

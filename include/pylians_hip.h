@@ -111,6 +111,15 @@ int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, doub
 int pyl_bk(const void* delta, int dims, double BoxSize, double k1, double k2, const double* theta, int ntheta,
            int mas_p, int prec, double* B, double* Q, double* k, double* Pk, void* stream);
 
+/* ---------------------------------------------------------------- instrumentation
+ * Builder-defined (no counterpart in the reference, whose only timing is printed wall time,
+ * MAS_library.pyx:71,112; Pk_library.pyx:269,382,425).  pyl_profile(1) resets the accumulators and
+ * records a hipEvent pair around every stage on the stream the stage runs on; pyl_profile_read
+ * writes "stage count total_ms" lines into buf (synchronising on the recorded events) and returns
+ * the full length of the text. */
+int pyl_profile(int enable);
+int pyl_profile_read(char* buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,8 +15,7 @@ in float64 arithmetic:
 * ``oracle.bk``   -- the ``Bk`` estimator, bispectrum_library.pyx:32-196.
 * ``oracle.adjoint`` -- FP64 analytic gradient of sum_k w_k P0(k) w.r.t. positions
   (builder-defined; no reference counterpart).
-* ``oracle.synth`` -- seeded synthetic inputs (uniform, Zel'dovich + RSD).
-* ``oracle.cpu_baseline`` -- ctypes front-end of the C restatements and of the
+* ``oracle.cpu`` -- ctypes front-end of the C restatements and of the
   reference's MAS_c.c built from source into ``oracle/_ref`` (``oracle/Makefile``).
 
 How each function is pinned (golden vectors, known-answer tests, brute force,

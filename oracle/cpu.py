@@ -95,3 +95,52 @@ def ref_mas_c(pos, number, BoxSize, MAS="CIC", W=None, threads=1):
     getattr(ref(), MAS)(_p(pos), _p(number), None if Wc is None else _p(Wc), pos.shape[0],
                         number.shape[0], pos.shape[1], float(BoxSize), int(threads))
     return number
+
+
+def cpu_reference_pipeline(pos, dims, BoxSize, axis, threads, MAS="CIC"):
+    """The reference's CPU path on this host, timed stage by stage (seconds):
+    MAS_c.c deposit with OpenMP `threads` (the reference's own C, oracle/_ref), overdensity
+    normalisation in float32 (Pk_snapshot.py:84-88), float32 r2c with scipy.fft (pocketfft, in
+    place of pyFFTW/FFTW, which this image lacks) on `threads` workers, and the serial mode loop
+    restated in C with the reference's float32 window multiply (Pk_library.pyx:314-381).
+    Returns (oracle.pk.Pk-like result, {stage: seconds})."""
+    import time
+
+    import scipy.fft
+
+    t = {}
+    t0 = time.perf_counter()
+    grid = np.zeros((dims, dims, dims), dtype=np.float32)
+    ref_mas_c(pos, grid, BoxSize, MAS, None, threads)
+    t1 = time.perf_counter()
+    mean = len(pos) * 1.0 / dims ** 3
+    grid /= np.float32(mean)
+    grid -= np.float32(1.0)
+    dk = scipy.fft.rfftn(grid, workers=threads)
+    t2 = time.perf_counter()
+    sums = pk_sums_c(dk, dims, axis, MAS, faithful_f32=True)
+    res = _pk.Pk(None, BoxSize, sums=sums, dims=dims)
+    t3 = time.perf_counter()
+    t["deposit"], t["normalise_fft"], t["modes"], t["total"] = t1 - t0, t2 - t1, t3 - t2, t3 - t0
+    return res, t
+
+
+def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_floor=False):
+    """FP64 oracle of the end-to-end path at full size (C deposit, float64 r2c, C float64 mode loop).
+    With with_floor, also returns the FP64 result on the float32 rounding of the exact overdensity
+    grid: the error floor of any float32 representation of delta."""
+    import scipy.fft
+
+    rho = deposit_f64(pos, dims, BoxSize, MAS)
+    rho /= len(pos) * 1.0 / dims ** 3
+    rho -= 1.0
+    d32 = rho.astype(np.float32) if with_floor else None
+    dk = scipy.fft.rfftn(rho, workers=workers)
+    del rho
+    o = _pk.Pk(None, BoxSize, sums=pk_sums_c(dk, dims, axis, MAS), dims=dims)
+    if not with_floor:
+        return o
+    del dk
+    dk = scipy.fft.rfftn(d32.astype(np.float64), workers=workers)
+    del d32
+    return o, _pk.Pk(None, BoxSize, sums=pk_sums_c(dk, dims, axis, MAS), dims=dims)

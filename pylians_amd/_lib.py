@@ -39,6 +39,8 @@ SIGNATURES = {
     "pyl_pk": [_vp, _i, _d, _i, _i, _i, _i, _vp, _vp, _vp],
     "pyl_pk_particles": [_vp, _vp, _i64, _i, _d, _i, _i, _i, _i, _vp, _vp],
     "pyl_bk": [_vp, _i, _d, _d, _d, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    "pyl_profile": [_i],
+    "pyl_profile_read": [_c.c_char_p, _i],
 }
 
 _lib = None
@@ -105,3 +107,20 @@ def empty(shape, dtype):
 
 def to_host(t):
     return t.detach().cpu().numpy()
+
+
+def profile(enable):
+    """Enable (and reset) or disable the library's per-stage HIP-event timers."""
+    check(lib().pyl_profile(1 if enable else 0))
+
+
+def profile_read():
+    """{stage: (count, total_ms)} from the library's stage timers."""
+    n = lib().pyl_profile_read(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib().pyl_profile_read(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        out[name] = (int(cnt), float(ms))
+    return out

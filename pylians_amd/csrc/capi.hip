@@ -7,6 +7,7 @@
 #include "deposit.hpp"
 #include "fft.hpp"
 #include "pk.hpp"
+#include "profile.hpp"
 #include "util.hpp"
 
 using namespace pyl;
@@ -159,12 +160,22 @@ static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int
     PkSums sums;
     rc = pk_sums_ws(g->d, &sums, nullptr, st);
     if (rc) return rc;
-    rc = fft_pass_z_r2c<R>(delta, spec, dims, P, affine, dmass, mass, st);
-    if (rc) return rc;
-    rc = fft_pass_cols<R>(spec, spec, dims, P, 1, false, (R)1, st);
-    if (rc) return rc;
-    rc = pk_x_pass_bin<R>(spec, keep, dims, P, axis, mas_p, flags, sums, st);
-    if (rc) return rc;
+    {
+        StageTimer t("fft_z_r2c", st);
+        rc = fft_pass_z_r2c<R>(delta, spec, dims, P, affine, dmass, mass, st);
+        if (rc) return rc;
+    }
+    {
+        StageTimer t("fft_y", st);
+        rc = fft_pass_cols<R>(spec, spec, dims, P, 1, false, (R)1, st);
+        if (rc) return rc;
+    }
+    {
+        StageTimer t("fft_x_bin", st);
+        rc = pk_x_pass_bin<R>(spec, keep, dims, P, axis, mas_p, flags, sums, st);
+        if (rc) return rc;
+    }
+    StageTimer t("finalize", st);
     return pk_finalize(g, sums, BoxSize, out, st);
 }
 
@@ -203,9 +214,11 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
         rc = weight_sum(W, n, dmass, st);
         if (rc) return rc;
     }
-    rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | (f64 ? DEP_GRID_F64 : 0) | (flags & (DEP_ATOMIC | DEP_BINNED)),
-                 grid, st);
-    if (rc) return rc;
+    {
+        rc = deposit(pos, W, n, 3, dims, BoxSize, scheme,
+                     DEP_ZERO | (f64 ? DEP_GRID_F64 : 0) | (flags & (DEP_ATOMIC | DEP_BINNED)), grid, st);
+        if (rc) return rc;
+    }
     const int pkflags = flags & PYL_PK_ALL;
     if (f64)
         return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, true,

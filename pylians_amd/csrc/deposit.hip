@@ -10,6 +10,7 @@
 #include <math.h>
 
 #include "deposit.hpp"
+#include "profile.hpp"
 #include "util.hpp"
 
 namespace pyl {
@@ -155,6 +156,7 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
     int64_t cells = 1;
     for (int a = 0; a < axes; ++a) cells *= dims;
     if (flags & DEP_ZERO) {
+        StageTimer t("grid_zero", st);
         int rz = zero_fill(grid, (size_t)cells * (f64 ? 8 : 4), st);
         if (rz) return rz;
     }
@@ -162,6 +164,7 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
     const double inv = dims / BoxSize;
     int rc = OK;
     if (n > 0) {
+        StageTimer t("deposit_atomic", st);
         rc = f64 ? dispatch_scheme<double>(pos, W, n, axes, dims, inv, mult, scheme, grid, st)
                  : dispatch_scheme<float>(pos, W, n, axes, dims, inv, mult, scheme, grid, st);
         if (rc) return rc;

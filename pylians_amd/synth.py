@@ -1,4 +1,4 @@
-"""Seeded synthetic inputs for the tests and the benchmark (TEST INFRASTRUCTURE ONLY).
+"""Seeded synthetic particle sets (uniform; Zel'dovich lattice with redshift-space distortions).
 
 * ``uniform``   -- positions uniform in [0, L), float32 (the CPU-runnable 128^3 case).
 * ``zeldovich`` -- a particle lattice q = (i+1/2) L/n displaced by the Zel'dovich
@@ -51,15 +51,64 @@ def zeldovich(n_side, BoxSize, seed, rms_disp=5.0, kc=0.5, sigma_v=0.0, axis=0,
     scale = rms_disp / np.sqrt(np.mean(np.sum(disp.astype(np.float64) ** 2, axis=1)))
     q = (np.arange(n, dtype=np.float64) + 0.5) * (L / n)
     pos = np.empty((n ** 3, 3), dtype=np.float32)
-    grid = np.meshgrid(q, q, q, indexing="ij")
     for a in range(3):
-        pos[:, a] = np.mod(grid[a].ravel() + scale * disp[:, a].astype(np.float64), L)
-    del disp, grid
+        shape = [1, 1, 1]
+        shape[a] = n
+        x = scale * disp[:, a].reshape(n, n, n).astype(np.float64)
+        x += q.reshape(shape)
+        pos[:, a] = np.mod(x, L).ravel()
+    del disp
     if sigma_v > 0.0:
         v = rng.normal(0.0, sigma_v, n ** 3).astype(np.float32)
         factor = np.float32((1.0 + redshift) / Hubble)
         x = pos[:, axis] + v * factor
         out = (x > np.float32(L)) | (x < np.float32(0.0))
         x[out] = np.mod(x[out] + np.float32(L), np.float32(L))
+        pos[:, axis] = x
+    return pos
+
+
+def zeldovich_device(n_side, BoxSize, seed, rms_disp=5.0, kc=0.5, sigma_v=0.0, axis=0, redshift=0.0,
+                     Hubble=100.0):
+    """Same construction as ``zeldovich`` generated on the GPU with torch (seeded Philox streams),
+    for the large benchmark sizes.  Returns a (n_side^3, 3) float32 device tensor in lattice order."""
+    import torch
+
+    n = int(n_side)
+    L = float(BoxSize)
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    white = torch.randn((n, n, n), generator=g, device=dev, dtype=torch.float32)
+    wk = torch.fft.rfftn(white)
+    del white
+    kF = 2.0 * np.pi / L
+    f = torch.fft.fftfreq(n, 1.0 / n, device=dev, dtype=torch.float32) * kF
+    fz = torch.arange(n // 2 + 1, device=dev, dtype=torch.float32) * kF
+    k2 = f[:, None, None] ** 2 + f[None, :, None] ** 2 + fz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    amp = torch.sqrt(torch.exp(-k2 / (kc * kc)) / k2) / k2
+    amp[0, 0, 0] = 0.0
+    wk = wk * (amp * 1j)
+    del amp, k2
+    pos = torch.empty((n ** 3, 3), device=dev, dtype=torch.float32)
+    disp = []
+    for ka in (f[:, None, None], f[None, :, None], fz[None, None, :]):
+        disp.append(torch.fft.irfftn(wk * ka, s=(n, n, n)))
+    del wk
+    ms = sum(torch.sum(d.double() ** 2) for d in disp) / n ** 3
+    scale = rms_disp / float(torch.sqrt(ms))
+    q = (torch.arange(n, device=dev, dtype=torch.float64) + 0.5) * (L / n)
+    for a in range(3):
+        shape = [1, 1, 1]
+        shape[a] = n
+        x = disp[a].double() * scale + q.reshape(shape)
+        pos[:, a] = torch.remainder(x, L).reshape(-1).float()
+        disp[a] = None
+    if sigma_v > 0.0:
+        v = torch.randn(n ** 3, generator=g, device=dev, dtype=torch.float32) * sigma_v
+        x = pos[:, axis] + v * np.float32((1.0 + redshift) / Hubble)
+        out = (x > L) | (x < 0.0)
+        x = torch.where(out, torch.remainder(x + L, L), x)
         pos[:, axis] = x
     return pos

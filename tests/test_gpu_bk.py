@@ -4,7 +4,7 @@ import pytest
 
 from oracle import bk as Bo
 from oracle import cpu as C
-from oracle import synth
+from pylians_amd import synth
 
 pytestmark = pytest.mark.gpu
 

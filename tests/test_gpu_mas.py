@@ -7,7 +7,7 @@ import torch
 
 from oracle import cpu as C
 from oracle import mas as M
-from oracle import synth
+from pylians_amd import synth
 from tests._cases import SCHEMES, places_ok, ref_mas_case
 
 pytestmark = pytest.mark.gpu

@@ -4,7 +4,7 @@ import pytest
 import torch
 
 from oracle import cpu as C
-from oracle import synth
+from pylians_amd import synth
 from tests._gpuref import assert_pk_close, oracle_pk
 
 pytestmark = pytest.mark.gpu

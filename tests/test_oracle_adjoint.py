@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle import adjoint as A
-from oracle import synth
+from pylians_amd import synth
 
 
 def test_gradient_matches_central_differences():
