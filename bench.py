@@ -65,8 +65,8 @@ def cpu_leg(pos_host, N, L, axis, gpu_pk, threads):
         "k3D": float(np.max(np.abs(gpu_pk.k3D - o.k3D) / o.k3D)),
         "bins_P0_within_1e-5": float(np.mean(np.abs(gpu_pk.Pk[:, 0] - P0) / np.abs(P0) <= 1e-5)),
         "bins_P2P4_within_1e-5_of_P0": float(np.mean(np.abs(gpu_pk.Pk[:, 1:] - o.Pk[:, 1:]) / np.abs(P0)[:, None] <= 1e-5)),
-        "f32_grid_floor_P0": float(np.max(np.abs(fl.Pk[:, 0] - P0) / np.abs(P0))),
-        "f32_grid_floor_P4_over_P0": float(np.max(np.abs(fl.Pk[:, 2] - o.Pk[:, 2]) / np.abs(P0))),
+        "f32_floor_P0": float(np.max(np.abs(fl.Pk[:, 0] - P0) / np.abs(P0))),
+        "f32_floor_P4_over_P0": float(np.max(np.abs(fl.Pk[:, 2] - o.Pk[:, 2]) / np.abs(P0))),
     }
     if "cpu_baseline" in out:
         err["cpu_reference_f32_P0"] = float(np.max(np.abs(res.Pk[:, 0] - P0) / np.abs(P0)))

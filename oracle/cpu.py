@@ -127,8 +127,8 @@ def cpu_reference_pipeline(pos, dims, BoxSize, axis, threads, MAS="CIC"):
 
 def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_floor=False):
     """FP64 oracle of the end-to-end path at full size (C deposit, float64 r2c, C float64 mode loop).
-    With with_floor, also returns the FP64 result on the float32 rounding of the exact overdensity
-    grid: the error floor of any float32 representation of delta."""
+    With with_floor, also returns the float32 floor: the exact overdensity rounded once to float32,
+    transformed with a float32 FFT (pocketfft single precision) and binned in float64."""
     import scipy.fft
 
     rho = deposit_f64(pos, dims, BoxSize, MAS)
@@ -141,6 +141,6 @@ def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_fl
     if not with_floor:
         return o
     del dk
-    dk = scipy.fft.rfftn(d32.astype(np.float64), workers=workers)
+    dk = scipy.fft.rfftn(d32, workers=workers).astype(np.complex128)
     del d32
     return o, _pk.Pk(None, BoxSize, sums=pk_sums_c(dk, dims, axis, MAS), dims=dims)

@@ -165,14 +165,15 @@ static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int
         rc = fft_pass_z_r2c<R>(delta, spec, dims, P, affine, dmass, mass, st);
         if (rc) return rc;
     }
+    const int last = pk_last_axis(axis);
     {
-        StageTimer t("fft_y", st);
-        rc = fft_pass_cols<R>(spec, spec, dims, P, 1, false, (R)1, st);
+        StageTimer t("fft_mid", st);
+        rc = fft_pass_cols<R>(spec, spec, dims, P, last == 0 ? 1 : 0, false, (R)1, st);
         if (rc) return rc;
     }
     {
-        StageTimer t("fft_x_bin", st);
-        rc = pk_x_pass_bin<R>(spec, keep, dims, P, axis, mas_p, flags, sums, st);
+        StageTimer t("fft_last_bin", st);
+        rc = pk_last_pass_bin<R>(spec, keep, dims, P, axis, mas_p, flags, sums, st);
         if (rc) return rc;
     }
     StageTimer t("finalize", st);
@@ -209,20 +210,24 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
     if (rc) return rc;
     double* dmass = nullptr;
     if (W) {
-        dmass = (double*)workspace(WS_AUX3, 64, &rc);
+        dmass = (double*)workspace(WS_MASS, 64, &rc);
         if (rc) return rc;
         rc = weight_sum(W, n, dmass, st);
         if (rc) return rc;
     }
-    {
-        rc = deposit(pos, W, n, 3, dims, BoxSize, scheme,
-                     DEP_ZERO | (f64 ? DEP_GRID_F64 : 0) | (flags & (DEP_ATOMIC | DEP_BINNED)), grid, st);
-        if (rc) return rc;
+    // binned path: the deposit writes delta = rho/rhobar - 1 itself (float64 before one rounding)
+    const bool binned = !(flags & DEP_ATOMIC) && binned_supported(3, dims);
+    if (binned) {
+        rc = deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st);
+    } else {
+        PYL_CHECK(!(flags & DEP_BINNED), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
+        rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | (f64 ? DEP_GRID_F64 : 0), grid, st);
     }
+    if (rc) return rc;
     const int pkflags = flags & PYL_PK_ALL;
     if (f64)
-        return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, true,
+        return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, !binned,
                                      dmass, (double)n, st);
-    return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, true, dmass,
-                                (double)n, st);
+    return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, !binned,
+                                dmass, (double)n, st);
 }

@@ -12,8 +12,10 @@
 // Per-mode arithmetic is in the working precision R; every shell sum is float64.
 #include <math.h>
 
+#include <functional>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "fft.hpp"
 #include "fft_engine.hpp"
@@ -50,57 +52,44 @@ __device__ __forceinline__ bool indep_mode(int kx, int ky, int kz, int middle, b
 }
 
 // ------------------------------------------------------------------ geometry
-__global__ void k_geom(int N, int axis, int kmax_par, int n1, int n3, unsigned long long* __restrict__ gn1,
-                       unsigned long long* __restrict__ gn2, unsigned long long* __restrict__ gn3,
-                       double* __restrict__ gk3) {
-    extern __shared__ double sm[];
-    double* k3 = sm;
-    unsigned* c3 = (unsigned*)(k3 + n3);
-    unsigned* c1 = c3 + n3;
-    for (int i = threadIdx.x; i < n3; i += blockDim.x) {
-        k3[i] = 0.0;
-        c3[i] = 0u;
-    }
-    for (int i = threadIdx.x; i < n1; i += blockDim.x) c1[i] = 0u;
-    __syncthreads();
+// Counts and |k| sums of the independent modes, computed once per (N, axis) on the host with
+// exact integer arithmetic; threads own slabs of kx and their histograms are summed in a fixed
+// order, so the float64 |k| sums are reproducible.
+static void geom_slab(int N, int axis, const PkDims& d, int x0, int x1, std::vector<unsigned long long>& n1,
+                      std::vector<unsigned long long>& n2, std::vector<unsigned long long>& n3,
+                      std::vector<double>& k3) {
     const int middle = N / 2;
     const bool even = (N % 2) == 0;
-    const long long Mz = middle + 1;
-    const long long total = (long long)N * N * Mz;
-    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
-         id += (long long)gridDim.x * blockDim.x) {
-        const int kzz = (int)(id % Mz);
-        const long long t = id / Mz;
-        const int kx = kfreq((int)(t / N), N), ky = kfreq((int)(t % N), N), kz = kzz;
-        if (!indep_mode(kx, ky, kz, middle, even)) continue;
-        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
-        const int bin = isqrt_i(k2);
-        atomicAdd(&c3[bin], 1u);
-        atomicAdd(&k3[bin], sqrt((double)k2));
-        int kpar, kper;
-        if (axis == 0) {
-            kpar = kx;
-            kper = isqrt_i((long long)ky * ky + (long long)kz * kz);
-        } else if (axis == 1) {
-            kpar = ky;
-            kper = isqrt_i((long long)kx * kx + (long long)kz * kz);
-        } else {
-            kpar = kz;
-            kper = isqrt_i((long long)kx * kx + (long long)ky * ky);
-        }
-        kpar = kpar < 0 ? -kpar : kpar;
-        if (k2 <= (long long)middle * middle) atomicAdd(&c1[kpar], 1u);
-        atomicAdd(&gn2[(long long)(kmax_par + 1) * kper + kpar], 1ull);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n3; i += blockDim.x) {
-        if (c3[i]) {
-            atomicAdd(&gn3[i], (unsigned long long)c3[i]);
-            atomicAdd(&gk3[i], k3[i]);
+    for (int kxx = x0; kxx < x1; ++kxx) {
+        const int kx = kxx > middle ? kxx - N : kxx;
+        for (int kyy = 0; kyy < N; ++kyy) {
+            const int ky = kyy > middle ? kyy - N : kyy;
+            for (int kz = 0; kz <= middle; ++kz) {
+                if (kz == 0 || (kz == middle && even)) {
+                    if (kx < 0) continue;
+                    if ((kx == 0 || (kx == middle && even)) && ky < 0) continue;
+                }
+                const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+                const int bin = isqrt_host(k2);
+                n3[bin] += 1;
+                k3[bin] += sqrt((double)k2);
+                int kpar, kper;
+                if (axis == 0) {
+                    kpar = kx;
+                    kper = isqrt_host((long long)ky * ky + (long long)kz * kz);
+                } else if (axis == 1) {
+                    kpar = ky;
+                    kper = isqrt_host((long long)kx * kx + (long long)kz * kz);
+                } else {
+                    kpar = kz;
+                    kper = isqrt_host((long long)kx * kx + (long long)ky * ky);
+                }
+                kpar = kpar < 0 ? -kpar : kpar;
+                if (k2 <= (long long)middle * middle) n1[kpar] += 1;
+                n2[(size_t)(d.kmax_par + 1) * kper + kpar] += 1;
+            }
         }
     }
-    for (int i = threadIdx.x; i < n1; i += blockDim.x)
-        if (c1[i]) atomicAdd(&gn1[i], (unsigned long long)c1[i]);
 }
 
 const PkGeometry* pk_geometry(int N, int axis, int* rc) {
@@ -134,22 +123,40 @@ const PkGeometry* pk_geometry(int N, int axis, int* rc) {
     g->n1 = (unsigned long long*)(base + bk);
     g->n2 = (unsigned long long*)(base + bk + b1);
     g->n3 = (unsigned long long*)(base + bk + b1 + b2);
-    hipError_t e = hipMemset(base, 0, b1 + b2 + b3 + bk);
-    size_t shm = sizeof(double) * d.n3() + sizeof(unsigned) * (d.n3() + d.n1());
-    hipLaunchKernelGGL(k_geom, dim3(1024), dim3(256), shm, 0, N, axis, d.kmax_par, d.n1(), d.n3(), g->n1, g->n2,
-                       g->n3, g->k3sum);
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    g->h_n1.resize(d.n1());
-    g->h_n2.resize(d.n2());
-    g->h_n3.resize(d.n3());
-    g->h_k3sum.resize(d.n3());
-    if (e == hipSuccess) e = hipMemcpy(g->h_n1.data(), g->n1, b1, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(g->h_n2.data(), g->n2, b2, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(g->h_n3.data(), g->n3, b3, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(g->h_k3sum.data(), g->k3sum, bk, hipMemcpyDeviceToHost);
+    g->h_n1.assign(d.n1(), 0);
+    g->h_n2.assign(d.n2(), 0);
+    g->h_n3.assign(d.n3(), 0);
+    g->h_k3sum.assign(d.n3(), 0.0);
+    {
+        unsigned nth = std::thread::hardware_concurrency();
+        nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
+        if ((int)nth > N) nth = N;
+        std::vector<std::vector<unsigned long long>> a1(nth, std::vector<unsigned long long>(d.n1(), 0)),
+            a2(nth, std::vector<unsigned long long>(d.n2(), 0)), a3(nth, std::vector<unsigned long long>(d.n3(), 0));
+        std::vector<std::vector<double>> ak(nth, std::vector<double>(d.n3(), 0.0));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nth; ++t) {
+            const int x0 = (int)((long long)N * t / nth), x1 = (int)((long long)N * (t + 1) / nth);
+            pool.emplace_back(geom_slab, N, axis, std::cref(d), x0, x1, std::ref(a1[t]), std::ref(a2[t]),
+                              std::ref(a3[t]), std::ref(ak[t]));
+        }
+        for (auto& th : pool) th.join();
+        for (unsigned t = 0; t < nth; ++t) {
+            for (int i = 0; i < d.n1(); ++i) g->h_n1[i] += a1[t][i];
+            for (int i = 0; i < d.n2(); ++i) g->h_n2[i] += a2[t][i];
+            for (int i = 0; i < d.n3(); ++i) {
+                g->h_n3[i] += a3[t][i];
+                g->h_k3sum[i] += ak[t][i];
+            }
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMemcpy(g->n1, g->h_n1.data(), b1, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->n2, g->h_n2.data(), b2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->n3, g->h_n3.data(), b3, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->k3sum, g->h_k3sum.data(), bk, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        set_error("geometry kernel failed: %s", hipGetErrorString(e));
+        set_error("geometry upload failed: %s", hipGetErrorString(e));
         *rc = E_LAUNCH;
         (void)hipFree(base);
         delete g;
@@ -184,144 +191,174 @@ static const double* window_table(int N, int p) {
 }
 const double* pk_window_table(int N, int p) { return window_table(N, p); }
 
-// ------------------------------------------------------------------ x pass + binning
+// ------------------------------------------------------------------ last pass + binning
 constexpr int isqrt_c(long long v) {
     long long r = 0;
     while ((r + 1) * (r + 1) <= v) ++r;
     return (int)r;
 }
-constexpr int hist_bytes(int N) {
-    return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * 4 * 8 + (N / 2 + 1) * 8;
+// LDS bytes of the histograms: 3-D (4 sums), 1-D, and the widest 2-D window (NL k_par rows)
+constexpr int hist_bytes(int N, int NL) {
+    return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * 4 * 8 + (N / 2 + 1) * 8 +
+           NL * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 8;
 }
 // lines per block for the binning kernel: as many as fit beside the histograms
 template <typename R, int N> constexpr int nl_bin() {
     int nl = Geo<R>::NL;
-    while (nl > 2 && N * nl * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + 8) + hist_bytes(N) > 150 * 1024)
+    while (nl > 2 && N * nl * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + 8) + hist_bytes(N, nl) > 150 * 1024)
         nl /= 2;
     return nl;
 }
 template <int N, int NL> constexpr int threads_bin() { return clamp_i(N * NL / 16, 64, 512); }
 
+// The last FFT pass of the forward transform, fused with the window deconvolution and the
+// binning.  The pass runs along x (last = 0; column (u = y, kz)) or y (last = 1; column
+// (u = x, kz)), chosen so that k_par is constant along each column unless the line of sight is z.
+// Persistent blocks own either one u (all kz blocks: "fixk = 0", k_par = |k_u|) or one kz block
+// and a range of u ("fixk = 1", k_par = kz), so their 2-D histogram window stays in LDS.
 template <typename R, int N, int NL, int T>
-__global__ __launch_bounds__(T) void k_x_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
-                                              long long P, int axis, int flags, int kmax_par, int n1, int n3,
-                                              const double* __restrict__ wing, const cplx<R>* __restrict__ twg,
-                                              double* __restrict__ g3, double* __restrict__ g1,
-                                              double* __restrict__ g2) {
-    constexpr int TPC = T / NL;  // threads per column
-    constexpr int E = (N + TPC - 1) / TPC;  // consecutive kx indices per thread in the epilogue
+__global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
+                                                 long long P, int last, int axis, int fixk, int nchunk, int flags,
+                                                 int kmax_par, int kmax_per, int n1, int n3,
+                                                 const double* __restrict__ wing, const cplx<R>* __restrict__ twg,
+                                                 double* __restrict__ g3, double* __restrict__ g1,
+                                                 double* __restrict__ g2) {
+    constexpr int TPC = T / NL;             // threads per column
+    constexpr int E = (N + TPC - 1) / TPC;  // consecutive elements per thread in the epilogue
     static_assert(T % NL == 0, "bad binning geometry");
     __shared__ cplx<R> s[N * NL];
     __shared__ cplx<R> tw[N];
     __shared__ double win[N];
     extern __shared__ double hs[];
-    double* h3 = hs;           // n3 * 4
-    double* h1 = hs + 4 * n3;  // n1
+    const int nper = kmax_per + 1;
+    double* h3 = hs;             // n3 * 4
+    double* h1 = hs + 4 * n3;    // n1
+    double* h2 = h1 + n1;        // NL * nper (fixk = 1) or nper (fixk = 0)
+    const int n2w = (fixk ? NL : 1) * nper;
     const int tid = threadIdx.x;
     for (int i = tid; i < N; i += T) {
         tw[i] = twg[i];
         win[i] = wing[i];
     }
-    for (int i = tid; i < 4 * n3 + n1; i += T) hs[i] = 0.0;
+    for (int i = tid; i < 4 * n3 + n1 + n2w; i += T) hs[i] = 0.0;
 
     const int middle = N / 2;
     const bool even = (N % 2) == 0;
-    const long long ncols = (long long)N * P;
-    const long long EST = ncols;
-    const long long nblk = (ncols + NL - 1) / NL;
+    const int nkzb = (int)((P + NL - 1) / NL);
+    const long long EST = last == 0 ? (long long)N * P : P;   // element stride along the pass axis
+    const long long UST = last == 0 ? P : (long long)N * P;   // stride of the outer coordinate u
     const int l = tid % NL;
     const int jrow = tid / NL;
     const bool do1 = flags & PK_1D, do2 = flags & PK_2D, doph = flags & PK_PHASE;
-
-    for (long long cb = blockIdx.x; cb < nblk; cb += gridDim.x) {
-        const long long c = cb * NL + l;
-        const bool valid = c < ncols;
-        __syncthreads();
-        for (int j = jrow; j < N; j += TPC) s[j * NL + l] = valid ? src[c + j * EST] : mk<R>(0, 0);
-        __syncthreads();
-        lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
-        if (!valid) continue;
-        const int kyy = (int)(c / P), kzz = (int)(c % P);
-        const int ky = kfreq(kyy, N), kz = kzz;
-        const int kyz2 = ky * ky + kz * kz;
-        const double cyz = win[kyy] * win[kzz];
-        const bool zplane = (kz == 0) || (kz == middle && even);
-        const int kper_fixed = isqrt_i(kyz2);
-        int cur3 = -1, cur1 = -1, cur2 = -1;
-        double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
-        const int j0 = jrow * E;
-#pragma unroll 4
-        for (int q = 0; q < E; ++q) {
-            const int j = j0 + q;
-            if (N % TPC != 0 && j >= N) break;
-            const int kx = kfreq(j, N);
-            cplx<R> v = s[j * NL + l];
-            const bool ind = !(zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0)));
-            if (!ind) {
-                if (keep) keep[c + j * EST] = v;
-                continue;
-            }
-            const R fac = (R)(win[j] * cyz);
-            v = scale(v, fac);
-            if (keep) keep[c + j * EST] = v;
-            const int k2 = kx * kx + kyz2;
-            const int bin = isqrt_i(k2);
-            int kpar = axis == 0 ? kx : (axis == 1 ? ky : kz);
-            const R mu2 = k2 == 0 ? (R)0 : (R)(kpar * kpar) / (R)k2;
-            const R d2 = v.x * v.x + v.y * v.y;
-            const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
-            const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
-            if (bin != cur3) {
-                if (cur3 >= 0) {
-                    atomicAdd(&h3[4 * cur3 + 0], a0);
-                    atomicAdd(&h3[4 * cur3 + 1], a2);
-                    atomicAdd(&h3[4 * cur3 + 2], a4);
-                    if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+    // work items of this block
+    int u0, u1, kzb0, kzb1;
+    if (fixk) {
+        const int kzb = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+        const int U = (N + nchunk - 1) / nchunk;
+        u0 = ch * U;
+        u1 = min(N, u0 + U);
+        kzb0 = kzb;
+        kzb1 = kzb + 1;
+    } else {
+        u0 = blockIdx.x;
+        u1 = u0 + 1;
+        kzb0 = 0;
+        kzb1 = nkzb;
+    }
+    const int kpar_u = fixk ? 0 : abs(kfreq(u0, N));
+    for (int u = u0; u < u1; ++u) {
+        for (int kzb = kzb0; kzb < kzb1; ++kzb) {
+            const int kzz = kzb * NL + l;
+            const bool valid = kzz < P;
+            const long long base = (long long)u * UST + kzz;
+            __syncthreads();
+            for (int j = jrow; j < N; j += TPC) s[j * NL + l] = valid ? src[base + j * EST] : mk<R>(0, 0);
+            __syncthreads();
+            lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
+            if (!valid) continue;
+            const int kz = kzz;
+            const int ku = kfreq(u, N);
+            const double cuz = win[u] * win[kzz];
+            const bool zplane = (kz == 0) || (kz == middle && even);
+            int cur3 = -1, cur2 = -1;
+            double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
+            const int j0 = jrow * E;
+            for (int q = 0; q < E; ++q) {
+                const int j = j0 + q;
+                if (N % TPC != 0 && j >= N) break;
+                const int kj = kfreq(j, N);
+                const int kx = last == 0 ? kj : ku;
+                const int ky = last == 0 ? ku : kj;
+                cplx<R> v = s[j * NL + l];
+                const bool ind = !(zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0)));
+                if (!ind) {
+                    if (keep) keep[base + j * EST] = v;
+                    continue;
                 }
-                cur3 = bin;
-                a0 = a2 = a4 = ap = 0.0;
-            }
-            a0 += (double)d2;
-            a2 += (double)(d2 * L2);
-            a4 += (double)(d2 * L4);
-            if (doph) {
-                const R ph = atan2(v.x, sqrt(d2));
-                ap += (double)(ph * ph);
-            }
-            kpar = kpar < 0 ? -kpar : kpar;
-            if (do1 && k2 <= middle * middle) {
-                if (kpar != cur1) {
-                    if (cur1 >= 0) atomicAdd(&h1[cur1], b1);
-                    cur1 = kpar;
-                    b1 = 0.0;
+                const R fac = (R)(win[j] * cuz);
+                v = scale(v, fac);
+                if (keep) keep[base + j * EST] = v;
+                const int k2 = kx * kx + ky * ky + kz * kz;
+                const int bin = isqrt_i(k2);
+                const int kps = axis == 0 ? kx : (axis == 1 ? ky : kz);
+                const R mu2 = k2 == 0 ? (R)0 : (R)(kps * kps) / (R)k2;
+                const R d2 = v.x * v.x + v.y * v.y;
+                const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
+                const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
+                if (bin != cur3) {
+                    if (cur3 >= 0) {
+                        atomicAdd(&h3[4 * cur3 + 0], a0);
+                        atomicAdd(&h3[4 * cur3 + 1], a2);
+                        atomicAdd(&h3[4 * cur3 + 2], a4);
+                        if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+                    }
+                    cur3 = bin;
+                    a0 = a2 = a4 = ap = 0.0;
                 }
-                b1 += (double)d2;
-            }
-            if (do2) {
-                const int kper = axis == 0 ? kper_fixed : isqrt_i(axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky);
-                const int i2 = (kmax_par + 1) * kper + kpar;
-                if (i2 != cur2) {
-                    if (cur2 >= 0) atomicAdd(&g2[cur2], b2);
-                    cur2 = i2;
-                    b2 = 0.0;
+                a0 += (double)d2;
+                a2 += (double)(d2 * L2);
+                a4 += (double)(d2 * L4);
+                if (doph) {
+                    const R ph = atan2(v.x, sqrt(d2));
+                    ap += (double)(ph * ph);
                 }
-                b2 += (double)d2;
+                if (do1 && k2 <= middle * middle) b1 += (double)d2;   // k_par is constant on the column
+                if (do2) {
+                    const int kper = isqrt_i(axis == 0 ? ky * ky + kz * kz
+                                                        : (axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
+                    const int w2 = (fixk ? l * nper : 0) + kper;
+                    if (w2 != cur2) {
+                        if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
+                        cur2 = w2;
+                        b2 = 0.0;
+                    }
+                    b2 += (double)d2;
+                }
             }
+            if (cur3 >= 0) {
+                atomicAdd(&h3[4 * cur3 + 0], a0);
+                atomicAdd(&h3[4 * cur3 + 1], a2);
+                atomicAdd(&h3[4 * cur3 + 2], a4);
+                if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+            }
+            if (do1 && b1 != 0.0) {
+                const int kpc = fixk ? kz : kpar_u;   // constant k_par of this column
+                atomicAdd(&h1[kpc], b1);
+            }
+            if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
         }
-        if (cur3 >= 0) {
-            atomicAdd(&h3[4 * cur3 + 0], a0);
-            atomicAdd(&h3[4 * cur3 + 1], a2);
-            atomicAdd(&h3[4 * cur3 + 2], a4);
-            if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
-        }
-        if (cur1 >= 0) atomicAdd(&h1[cur1], b1);
-        if (cur2 >= 0) atomicAdd(&g2[cur2], b2);
     }
     __syncthreads();
     for (int i = tid; i < 4 * n3; i += T)
         if (hs[i] != 0.0) atomicAdd(&g3[i], hs[i]);
     for (int i = tid; i < n1; i += T)
         if (h1[i] != 0.0) atomicAdd(&g1[i], h1[i]);
+    for (int i = tid; i < n2w; i += T) {
+        if (h2[i] == 0.0) continue;
+        const int kper = i % nper;
+        const int kpar = fixk ? kzb0 * NL + i / nper : kpar_u;
+        if (kpar <= kmax_par) atomicAdd(&g2[(long long)(kmax_par + 1) * kper + kpar], h2[i]);
+    }
 }
 
 static int num_cus() {
@@ -335,22 +372,31 @@ static int num_cus() {
     return n;
 }
 
+int pk_last_axis(int axis) { return axis == 0 ? 1 : 0; }
+
 template <typename R, int N>
-static int launch_x_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
-                        const PkSums& sums, hipStream_t st) {
+static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
+                           const PkSums& sums, hipStream_t st) {
     constexpr int NL = nl_bin<R, N>();
     constexpr int T = threads_bin<N, NL>();
     const PkDims d = pk_dims(N, P);
     const cplx<R>* tw = twiddles<R>(N);
     const double* win = window_table(N, p);
     PYL_CHECK(tw && win, E_ALLOC, "table allocation failed");
-    const long long ncols = (long long)N * P;
-    const long long nblk = (ncols + NL - 1) / NL;
-    long long grid = (long long)num_cus() * 4;
-    if (grid > nblk) grid = nblk;
-    size_t shm = sizeof(double) * (4 * (size_t)d.n3() + d.n1());
-    hipLaunchKernelGGL((k_x_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P, axis,
-                       flags, d.kmax_par, d.n1(), d.n3(), win, tw, sums.s3, sums.s1, sums.s2);
+    const int last = pk_last_axis(axis);
+    const int fixk = axis == 2 ? 1 : 0;
+    const int nkzb = (int)((P + NL - 1) / NL);
+    int nchunk = 1, grid = N;
+    if (fixk) {
+        nchunk = (num_cus() * 4 + nkzb - 1) / nkzb;
+        if (nchunk > N) nchunk = N;
+        grid = nkzb * nchunk;
+    }
+    const int nper = d.kmax_per + 1;
+    size_t shm = sizeof(double) * (4 * (size_t)d.n3() + d.n1() + (size_t)(fixk ? NL : 1) * nper);
+    hipLaunchKernelGGL((k_last_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P,
+                       last, axis, fixk, nchunk, flags, d.kmax_par, d.kmax_per, d.n1(), d.n3(), win, tw, sums.s3,
+                       sums.s1, sums.s2);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -360,12 +406,12 @@ static int launch_x_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis,
     X(320) X(384) X(512) X(640) X(768) X(1024)
 
 template <typename R>
-int pk_x_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
-                  const PkSums& sums, hipStream_t st) {
+int pk_last_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
+                     const PkSums& sums, hipStream_t st) {
     switch (N) {
 #define PYL_CASE(n) \
     case n:         \
-        return launch_x_bin<R, n>(spec, keep, P, axis, mas_p, flags, sums, st);
+        return launch_last_bin<R, n>(spec, keep, P, axis, mas_p, flags, sums, st);
         PYL_PK_SIZES(PYL_CASE)
 #undef PYL_CASE
         default:
@@ -373,10 +419,10 @@ int pk_x_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis
             return E_UNSUPPORTED;
     }
 }
-template int pk_x_pass_bin<float>(const cplx<float>*, cplx<float>*, int, int64_t, int, int, int, const PkSums&,
-                                  hipStream_t);
-template int pk_x_pass_bin<double>(const cplx<double>*, cplx<double>*, int, int64_t, int, int, int, const PkSums&,
-                                   hipStream_t);
+template int pk_last_pass_bin<float>(const cplx<float>*, cplx<float>*, int, int64_t, int, int, int, const PkSums&,
+                                     hipStream_t);
+template int pk_last_pass_bin<double>(const cplx<double>*, cplx<double>*, int, int64_t, int, int, int, const PkSums&,
+                                      hipStream_t);
 
 // ------------------------------------------------------------------ finalize
 __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxSize,

@@ -39,12 +39,14 @@ struct PkSums {
 
 enum PkFlags { PK_1D = 1, PK_2D = 2, PK_PHASE = 4 };
 
-// Last FFT pass (along x) of a half spectrum that already went through passes Z and Y,
-// fused with the window deconvolution and the binning of the independent modes.
+// Axis of the last forward FFT pass for a line of sight `axis`: 1 (y) for axis 0, else 0 (x).
+// The half spectrum must have gone through pass Z and the column pass along the other axis.
+int pk_last_axis(int axis);
+// That last pass fused with the window deconvolution and the binning of the independent modes.
 // keep (nullable) receives the window-corrected spectrum (Pk_library.pyx:354-355, 423).
 template <typename R>
-int pk_x_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
-                  const PkSums& sums, hipStream_t st);
+int pk_last_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
+                     const PkSums& sums, hipStream_t st);
 
 // Normalisation of Pk_library.pyx:387-421 on the device.  out layout (float64):
 //   [k3D (kmax) | P0 (kmax) | P2 (kmax) | P4 (kmax) | Pkphase (kmax) | Nmodes3D (kmax) |

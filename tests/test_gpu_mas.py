@@ -120,3 +120,62 @@ def test_mas_c_host_entry_points(gpu):
         if C.ref_available():
             r = C.ref_mas_c(pos, np.zeros((32,) * 3, np.float32), 50.0, s, W, threads=4)
             assert np.allclose(g, r, rtol=1e-4, atol=1e-5)
+
+
+def _dev_deposit(pos, W, dims, L, scheme, flags, start=None, f64=False):
+    from pylians_amd import _lib
+    p = torch.from_numpy(np.ascontiguousarray(pos)).cuda()
+    w = None if W is None else torch.from_numpy(np.ascontiguousarray(W)).cuda()
+    g = torch.from_numpy(np.zeros((dims,) * 3, np.float64 if f64 else np.float32) if start is None else start.copy()).cuda()
+    _lib.check(_lib.lib().pyl_mas_deposit(_lib.ptr(p), _lib.ptr(w), len(pos), 3, dims, float(L), scheme,
+                                          flags | (_lib.GRID_F64 if f64 else 0), _lib.ptr(g), _lib.stream()))
+    return g.cpu().numpy()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_binned_is_correctly_rounded(gpu, scheme, weighted):
+    """The cell-binned deposit accumulates in float64 (LDS) and rounds each cell once."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(31)
+    n, dims, L = 300000, 64, 200.0
+    pos = synth.zeldovich(64, L, seed=4, rms_disp=3.0, sigma_v=100.0)[:n] if not weighted else \
+        rng.random((n, 3), dtype=np.float32) * np.float32(L)
+    W = (rng.random(len(pos), dtype=np.float32) + np.float32(0.1)) if weighted else None
+    o = C.deposit_f64(pos, dims, L, scheme, W)
+    s = _lib.SCHEMES[scheme]
+    g = _dev_deposit(pos, W, dims, L, s, _lib.BINNED | _lib.ZERO)
+    assert np.array_equal(g, o.astype(np.float32)) or np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8
+    ga = _dev_deposit(pos, W, dims, L, s, _lib.ATOMIC | _lib.ZERO)
+    assert np.all(np.abs(ga - o) <= 1e-5 * np.abs(o) + 1e-12)
+    # accumulate into an existing grid: one rounding of old + sum
+    start = rng.random((dims,) * 3).astype(np.float32)
+    gacc = _dev_deposit(pos, W, dims, L, s, _lib.BINNED, start=start)
+    assert np.max(np.abs(gacc - (start + o)) / (start + o)) <= 6.5e-8
+    g64 = _dev_deposit(pos, W, dims, L, s, _lib.BINNED | _lib.ZERO, f64=True)
+    assert np.allclose(g64, o, rtol=1e-13, atol=1e-13)
+
+
+def test_binned_overflow_path_clustered(gpu):
+    """All particles inside a few cells: buckets overflow and the exact-count path takes over."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(5)
+    n, dims, L = 400000, 64, 64.0
+    pos = np.concatenate([rng.random((n // 2, 3), dtype=np.float32) * np.float32(2.0) + np.float32(10.0),
+                          rng.random((n // 2, 3), dtype=np.float32) * np.float32(L)]).astype(np.float32)
+    for scheme in (2, 4):
+        name = {2: "CIC", 4: "PCS"}[scheme]
+        o = C.deposit_f64(pos, dims, L, name)
+        g = _dev_deposit(pos, None, dims, L, scheme, _lib.BINNED | _lib.ZERO)
+        assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8
+        assert abs(np.sum(g, dtype=np.float64) / n - 1.0) < 1e-6
+
+
+def test_binned_periodic_edges(gpu):
+    from pylians_amd import _lib
+    dims, L = 32, 32.0
+    pos = np.array([[0.0, 0.0, 0.0], [32.0, 32.0, 32.0], [31.99, 0.01, 31.5], [15.999, 16.0, 16.001]] * 50, np.float32)
+    for s, name in ((1, "NGP"), (2, "CIC"), (3, "TSC"), (4, "PCS")):
+        o = C.deposit_f64(pos, dims, L, name)
+        g = _dev_deposit(pos, None, dims, L, s, _lib.BINNED | _lib.ZERO)
+        assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, name

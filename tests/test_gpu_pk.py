@@ -5,7 +5,7 @@ import torch
 
 from oracle import cpu as C
 from pylians_amd import synth
-from tests._gpuref import assert_pk_close, oracle_pk
+from tests._gpuref import assert_pk_close, f32_floor_pk, oracle_pk
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,7 @@ def test_pk_float32_parity(PKL, N, axis):
     delta = density_field(N, L, seed=N + axis)
     g = PKL.Pk(delta, L, axis=axis, MAS="CIC", threads=4)
     o = oracle_pk(delta, L, axis, "CIC")
-    errs = assert_pk_close(g, o, rtol=1e-5)
+    errs = assert_pk_close(g, o, rtol=1e-5, floor=f32_floor_pk(delta, L, axis, "CIC"))
     print("N=%d axis=%d max rel err vs FP64 oracle: %s" % (N, axis, errs))
 
 
@@ -39,7 +39,7 @@ def test_pk_windows(PKL, MAS):
     delta = density_field(N, L, seed=3, MAS=MAS if MAS != "None" else "CIC")
     g = PKL.Pk(delta, L, axis=2, MAS=MAS)
     o = oracle_pk(delta, L, 2, MAS)
-    assert_pk_close(g, o, rtol=1e-5)
+    assert_pk_close(g, o, rtol=1e-5, floor=f32_floor_pk(delta, L, 2, MAS))
 
 
 @pytest.mark.parametrize("N", [32, 64])
@@ -65,7 +65,7 @@ def test_device_input_and_odd_free_sizes(PKL):
     delta = density_field(N, L, seed=4)
     g = PKL.Pk(torch.from_numpy(delta).cuda(), L, axis=0, MAS="CIC")
     o = oracle_pk(delta, L, 0, "CIC")
-    assert_pk_close(g, o, rtol=1e-5)
+    assert_pk_close(g, o, rtol=1e-5, floor=f32_floor_pk(delta, L, 0, "CIC"))
 
 
 def test_plane_wave_closed_form(PKL):
