@@ -111,6 +111,46 @@ int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, doub
 int pyl_bk(const void* delta, int dims, double BoxSize, double k1, double k2, const double* theta, int ntheta,
            int mas_p, int prec, double* B, double* Q, double* k, double* Pk, void* stream);
 
+/* ---------------------------------------------------------------- batches
+ * Builder-defined (the reference processes one field per call, Pk/Pk.py:17-27).  A batch is an
+ * array of independent particles -> P(k) jobs of any sizes, run back to back on one stream; the
+ * jobs share the library workspaces.  pyl_graph_create runs the batch once (sizing every workspace
+ * and plan), then captures it into a hipGraph; pyl_graph_launch replays the whole batch with one
+ * launch.  Inputs are read and outputs written at the device addresses given in the jobs. */
+typedef struct {
+    const float* pos;   /* (n, 3) float32, device */
+    const float* W;     /* (n,) float32 or NULL */
+    int64_t n;
+    int dims;
+    double BoxSize;
+    int scheme;         /* 1..4, also the window exponent */
+    int axis;
+    int prec;           /* PYL_F32 / PYL_F64 */
+    int flags;          /* PYL_PK_* | PYL_ATOMIC / PYL_BINNED */
+    double* out;        /* device, pyl_pk_layout(dims) values */
+} pyl_pk_job;
+
+int pyl_pk_batch(const pyl_pk_job* jobs, int njobs, void* stream);
+int pyl_graph_create(const pyl_pk_job* jobs, int njobs, void* stream, void** graph);
+int pyl_graph_launch(void* graph, void* stream);
+int pyl_graph_destroy(void* graph);
+
+/* ---------------------------------------------------------------- differentiation
+ * Builder-defined: the reference has no adjoint.  Gradient of L = sum_{b<nb} w[b] P0[b] (P0 of
+ * pyl_pk_particles with `scheme`, no weights, line of sight irrelevant for P0; bin b is the b-th
+ * output bin, |k| in [b+1, b+2) kF) with respect to the particle positions.  w: device float64
+ * (nb); grad: device float32 (n, 3); out: device float64 pyl_pk_layout vector of the forward pass
+ * (P0 etc., so L can be formed).  Nearest reference forms: CIC_interp (MAS_library.pyx:558-599),
+ * the transpose of the CIC scatter, and IFFT3Dr_f (Pk_library.pyx:152-165). */
+int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxSize, int scheme, const double* w, int nb,
+                   float* grad, double* out, void* stream);
+
+/* Replaces MAS_library.CIC_interp(density, BoxSize, pos, den) (MAS_library.pyx:558-599) for scheme 2,
+ * and generalises it to NGP/TSC/PCS: out[p] = sum over the stencil cells of density * weight.
+ * density: device float32 (dims^3); pos: device float32 (n, 3); out: device float32 (n). */
+int pyl_interp(const float* density, int dims, double BoxSize, const float* pos, int64_t n, int scheme, float* out,
+               void* stream);
+
 /* ---------------------------------------------------------------- instrumentation
  * Builder-defined (no counterpart in the reference, whose only timing is printed wall time,
  * MAS_library.pyx:71,112; Pk_library.pyx:269,382,425).  pyl_profile(1) resets the accumulators and

@@ -157,6 +157,15 @@ PCSc2D, PCSWc2D = _c_path("PCS")
 
 
 def CIC_interp(density, BoxSize, pos, den):
-    """CIC interpolation of a 3-D grid at particle positions (MAS_library.pyx:558-599)."""
-    from . import adjoint
-    adjoint.cic_interp(density, BoxSize, pos, den)
+    """CIC interpolation of a 3-D float32 grid at particle positions into `den` (MAS_library.pyx:558-599)."""
+    g = _lib.to_dev(density, torch.float32)
+    p = _lib.to_dev(pos, torch.float32)
+    n = int(p.shape[0])
+    out = den if (_lib.is_dev(den) and den.dtype == torch.float32 and den.is_contiguous()) else _lib.empty((n,), torch.float32)
+    _lib.check(_lib.lib().pyl_interp(_lib.ptr(g), int(g.shape[0]), float(BoxSize), _lib.ptr(p), n, 2,
+                                     _lib.ptr(out), _lib.stream()))
+    if out is not den:
+        if _lib.is_dev(den):
+            den.copy_(out)
+        else:
+            den[...] = _lib.to_host(out)

@@ -39,11 +39,23 @@ SIGNATURES = {
     "pyl_pk": [_vp, _i, _d, _i, _i, _i, _i, _vp, _vp, _vp],
     "pyl_pk_particles": [_vp, _vp, _i64, _i, _d, _i, _i, _i, _i, _vp, _vp],
     "pyl_bk": [_vp, _i, _d, _d, _d, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    "pyl_pk_adjoint": [_vp, _i64, _i, _d, _i, _vp, _i, _vp, _vp, _vp],
+    "pyl_interp": [_vp, _i, _d, _vp, _i64, _i, _vp, _vp],
+    "pyl_pk_batch": [_vp, _i, _vp],
+    "pyl_graph_create": [_vp, _i, _vp, _vp],
+    "pyl_graph_launch": [_vp, _vp],
+    "pyl_graph_destroy": [_vp],
     "pyl_profile": [_i],
     "pyl_profile_read": [_c.c_char_p, _i],
 }
 
 _lib = None
+
+
+class PkJob(ctypes.Structure):
+    """pyl_pk_job of include/pylians_hip.h"""
+    _fields_ = [("pos", _vp), ("W", _vp), ("n", _i64), ("dims", _i), ("BoxSize", _d), ("scheme", _i),
+                ("axis", _i), ("prec", _i), ("flags", _i), ("out", _vp)]
 
 
 class PylError(RuntimeError):

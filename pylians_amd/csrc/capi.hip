@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include "../../include/pylians_hip.h"
+#include "adjoint.hpp"
 #include "common.hpp"
 #include "deposit.hpp"
 #include "fft.hpp"
@@ -19,6 +20,7 @@ static_assert(PYL_PK_1D == PK_1D && PYL_PK_2D == PK_2D && PYL_PK_PHASE == PK_PHA
 
 namespace pyl {
 int weight_sum(const float* W, int64_t n, double* out, hipStream_t st);
+const double* pk_window_table(int N, int p);
 }
 
 extern "C" int pyl_abi_version(void) { return PYL_ABI_VERSION; }
@@ -230,4 +232,123 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
                                      dmass, (double)n, st);
     return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, !binned,
                                 dmass, (double)n, st);
+}
+
+// ------------------------------------------------------------------ adjoint and interpolation
+extern "C" int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxSize, int scheme, const double* w,
+                              int nb, float* grad, double* out, void* stream) {
+    int rc = check_fft(dims, PYL_F32);
+    if (rc) return rc;
+    PYL_CHECK(scheme >= 1 && scheme <= 4, E_ARG, "unknown mass-assignment scheme %d", scheme);
+    PYL_CHECK(n > 0 && pos && w && grad && out, E_ARG, "bad arguments");
+    const PkDims d = pk_dims(dims, half_len(dims));
+    PYL_CHECK(nb >= 0 && nb <= d.kmax, E_ARG, "nb must be in [0, kmax = %d]", d.kmax);
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t P = half_len(dims);
+    const size_t cells = (size_t)dims * dims * dims;
+    float* grid = (float*)workspace(WS_GRID, cells * 4, &rc);
+    if (rc) return rc;
+    cplx<float>* keep = (cplx<float>*)workspace(WS_SPEC2, sizeof(cplx<float>) * (size_t)dims * dims * P, &rc);
+    if (rc) return rc;
+    // forward: delta, spectrum (kept, window-corrected) and the binned outputs (out = P0 etc.)
+    const bool binned = binned_supported(3, dims);
+    {
+        StageTimer t("adj_forward_deposit", st);
+        rc = binned ? deposit_binned(pos, nullptr, n, dims, BoxSize, scheme, false, 2, 1.0, grid, nullptr, (double)n, st)
+                    : deposit(pos, nullptr, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC, grid, st);
+        if (rc) return rc;
+    }
+    rc = pk_from_delta<float>(grid, dims, BoxSize, 0, scheme, 0, out, keep, !binned, nullptr, (double)n, st);
+    if (rc) return rc;
+    const PkGeometry* g = pk_geometry(dims, 0, &rc);
+    if (rc) return rc;
+    const double* win = pk_window_table(dims, scheme);
+    PYL_CHECK(win, E_ALLOC, "window table allocation failed");
+    const double rhobar = (double)n / ((double)dims * dims * dims);
+    {
+        StageTimer t("adj_grad_spectrum", st);
+        rc = grad_spectrum<float>(keep, dims, P, w, nb, g->n3, BoxSize, win, 1.0 / rhobar, st);
+        if (rc) return rc;
+    }
+    {
+        StageTimer t("adj_c2r", st);
+        cplx<float>* ws = (cplx<float>*)workspace(WS_SPEC, sizeof(cplx<float>) * (size_t)dims * dims * P, &rc);
+        if (rc) return rc;
+        rc = fft3d_c2r<float>(keep, grid, ws, dims, P, 1.0f, st);
+        if (rc) return rc;
+    }
+    StageTimer t("adj_gather", st);
+    return gather_grad(grid, dims, BoxSize, pos, n, scheme, 1.0, grad, st);
+}
+
+extern "C" int pyl_interp(const float* density, int dims, double BoxSize, const float* pos, int64_t n, int scheme,
+                          float* out, void* stream) {
+    PYL_CHECK(dims > 0 && BoxSize > 0 && density && (n == 0 || (pos && out)), E_ARG, "bad arguments");
+    PYL_CHECK(scheme >= 1 && scheme <= 4, E_ARG, "unknown mass-assignment scheme %d", scheme);
+    return interp(density, dims, BoxSize, pos, n, scheme, out, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ batches and graphs
+extern "C" int pyl_pk_batch(const pyl_pk_job* jobs, int njobs, void* stream) {
+    PYL_CHECK(njobs >= 0 && (njobs == 0 || jobs), E_ARG, "bad job array");
+    for (int i = 0; i < njobs; ++i) {
+        const pyl_pk_job& j = jobs[i];
+        int rc = pyl_pk_particles(j.pos, j.W, j.n, j.dims, j.BoxSize, j.scheme, j.axis, j.prec, j.flags, j.out, stream);
+        if (rc) return rc;
+    }
+    return OK;
+}
+
+namespace {
+struct GraphHandle {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+extern "C" int pyl_graph_create(const pyl_pk_job* jobs, int njobs, void* stream, void** graph) {
+    PYL_CHECK(graph, E_ARG, "null graph handle");
+    hipStream_t st = (hipStream_t)stream;
+    PYL_CHECK(st != nullptr, E_ARG, "graph capture needs a non-default stream");
+    // warm run: sizes every workspace, builds twiddle / window / geometry tables
+    int rc = pyl_pk_batch(jobs, njobs, stream);
+    if (rc) return rc;
+    PYL_HIP(hipStreamSynchronize(st));
+    GraphHandle* h = new GraphHandle();
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        delete h;
+        PYL_HIP(e);
+    }
+    rc = pyl_pk_batch(jobs, njobs, stream);
+    e = hipStreamEndCapture(st, &h->graph);
+    if (rc || e != hipSuccess) {
+        if (h->graph) (void)hipGraphDestroy(h->graph);
+        delete h;
+        if (rc) return rc;
+        PYL_HIP(e);
+    }
+    e = hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(h->graph);
+        delete h;
+        PYL_HIP(e);
+    }
+    *graph = h;
+    return OK;
+}
+
+extern "C" int pyl_graph_launch(void* graph, void* stream) {
+    PYL_CHECK(graph, E_ARG, "null graph");
+    PYL_HIP(hipGraphLaunch(((GraphHandle*)graph)->exec, (hipStream_t)stream));
+    return OK;
+}
+
+extern "C" int pyl_graph_destroy(void* graph) {
+    if (!graph) return OK;
+    GraphHandle* h = (GraphHandle*)graph;
+    if (h->exec) PYL_HIP(hipGraphExecDestroy(h->exec));
+    if (h->graph) PYL_HIP(hipGraphDestroy(h->graph));
+    delete h;
+    return OK;
 }
