@@ -74,6 +74,102 @@ def cpu_leg(pos_host, N, L, axis, gpu_pk, threads):
     return out
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def workload_batch(args):
+    """64 seeded realizations: N_p log-uniform in [64^3, 256^3], N in {128,192,256,384},
+    L in [250, 1000], uniform positions, CIC + P0, one hipGraph launch per step."""
+    import torch
+
+    from pylians_amd import _lib, batch, synth
+    rng = np.random.default_rng(100)
+    jobs = []
+    for r in range(args.batch):
+        n_p = int(round(np.exp(rng.uniform(np.log(64 ** 3), np.log(256 ** 3)))))
+        N = int(rng.choice([128, 192, 256, 384]))
+        L = float(rng.uniform(250.0, 1000.0))
+        g = torch.Generator(device="cuda")
+        g.manual_seed(100 + r)
+        pos = torch.rand((n_p, 3), generator=g, device="cuda", dtype=torch.float32) * np.float32(L)
+        jobs.append({"pos": pos, "BoxSize": L, "dims": N, "MAS": "CIC", "axis": 0})
+    b = batch.PkBatch(jobs, flags=0)
+    for _ in range(args.warmup):
+        b.run()
+    b.stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(b.stream):
+        e0.record()
+        for _ in range(args.steps):
+            b.run()
+        e1.record()
+    b.stream.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    n_tot = b.n_particles
+    cells = sum(j["dims"] ** 3 for j in jobs)
+    nh = sum(j["dims"] ** 2 * (j["dims"] // 2 + 1) for j in jobs)
+    bytes_e2e = 12 * n_tot + 8 * cells + 16 * nh
+    return {"metric": "Mparticles/s batched CIC+P0, %d uneven realizations in one hipGraph launch" % args.batch,
+            "value": round(n_tot / (ms * 1e-3) / 1e6, 1), "unit": "Mparticles/s", "ms_per_step": round(ms, 4),
+            "grids_per_s": round(args.batch / (ms * 1e-3), 1), "e2e_hbm_frac": round(bytes_e2e / (ms * 1e-3) / 8e12, 4),
+            "config": {"workload": "batch%d_uniform_cic_p0" % args.batch, "n_particles_total": n_tot,
+                       "grids": sorted(set(j["dims"] for j in jobs))}, "dtype": "f32", "steps": args.steps}
+
+
+def workload_adjoint(args):
+    """256^3 Zel'dovich particles (no RSD), CIC onto 256^3, L = 500: forward P0 + dLoss/dpos."""
+    import torch
+
+    from pylians_amd import autograd, synth
+    pos = synth.zeldovich_device(256, 500.0, seed=3, rms_disp=5.0)
+    w = torch.tensor(np.random.default_rng(3).random(64), device="cuda")
+    ms = _timed(lambda: autograd.grad_loss(pos, 500.0, 256, w), args.steps, args.warmup)
+    n = int(pos.shape[0])
+    return {"metric": "Mparticles/s forward P0 + adjoint dL/dpos, 256^3 CIC", "value": round(n / (ms * 1e-3) / 1e6, 1),
+            "unit": "Mparticles/s", "ms_per_step": round(ms, 4), "dtype": "f32", "steps": args.steps,
+            "config": {"workload": "zeldovich256_cic_adjoint_64bins", "n_particles": n, "grid": 256, "BoxSize": 500.0}}
+
+
+def workload_bispectrum(args):
+    """16 realizations of 512^3 Zel'dovich particles on a 256^3 grid, CIC, Bk(k1=0.1, k2=0.2, 20 theta)."""
+    import torch
+
+    from pylians_amd import Pk_library, _lib, synth
+    N, L = 256, 1000.0
+    theta = np.linspace(0, np.pi, 20)
+    deltas = []
+    for r in range(args.realizations):
+        pos = synth.zeldovich_device(512, L, seed=200 + r, rms_disp=5.0)
+        g = torch.zeros((N, N, N), dtype=torch.float32, device="cuda")
+        _lib.check(_lib.lib().pyl_mas_deposit(_lib.ptr(pos), None, int(pos.shape[0]), 3, N, L, 2, _lib.ZERO,
+                                              _lib.ptr(g), _lib.stream()))
+        g /= g.double().mean().float()
+        g -= 1.0
+        deltas.append(g)
+        del pos
+    Pk_library.Bk(deltas[0], L, 0.1, 0.2, theta)       # builds the cached indicator sums
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for d in deltas:
+        Pk_library.Bk(d, L, 0.1, 0.2, theta)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / len(deltas)
+    return {"metric": "Bk per realization (20 theta bins, 256^3, indicator sums cached)", "value": round(ms, 3),
+            "unit": "ms/realization", "higher_is_better": False, "dtype": "f32",
+            "config": {"workload": "bk_zeldovich512_grid256_20theta", "realizations": len(deltas)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,10 +181,17 @@ def main():
     ap.add_argument("--deposit", default=None, choices=[None, "atomic", "binned"])
     ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 oracle check")
+    ap.add_argument("--workload", default="zeldovich512", choices=["zeldovich512", "batch", "adjoint", "bispectrum"])
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--realizations", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "zeldovich512":
+        fn = {"batch": workload_batch, "adjoint": workload_adjoint, "bispectrum": workload_bispectrum}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
 
     import torch
 

@@ -151,6 +151,16 @@ int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxSize, int sc
 int pyl_interp(const float* density, int dims, double BoxSize, const float* pos, int64_t n, int scheme, float* out,
                void* stream);
 
+/* ---------------------------------------------------------------- deposit autotuner
+ * Builder-defined.  Without PYL_ATOMIC / PYL_BINNED, pyl_pk_particles picks the deposit kernel per
+ * (dims, scheme, prec, weights, floor(log2 n)) by timing both once (outside graph capture) and
+ * caching the faster.  pyl_tune(0) disables timing (binned is used where supported);
+ * pyl_tune_export / pyl_tune_import move the cache as text lines "dims scheme prec hasw nlog choice"
+ * (choice 0 = binned, 1 = atomic); export returns the full text length. */
+int pyl_tune(int enable);
+int pyl_tune_export(char* buf, int len);
+int pyl_tune_import(const char* text);
+
 /* ---------------------------------------------------------------- instrumentation
  * Builder-defined (no counterpart in the reference, whose only timing is printed wall time,
  * MAS_library.pyx:71,112; Pk_library.pyx:269,382,425).  pyl_profile(1) resets the accumulators and

@@ -45,6 +45,9 @@ SIGNATURES = {
     "pyl_graph_create": [_vp, _i, _vp, _vp],
     "pyl_graph_launch": [_vp, _vp],
     "pyl_graph_destroy": [_vp],
+    "pyl_tune": [_i],
+    "pyl_tune_export": [_c.c_char_p, _i],
+    "pyl_tune_import": [_c.c_char_p],
     "pyl_profile": [_i],
     "pyl_profile_read": [_c.c_char_p, _i],
 }

@@ -109,6 +109,21 @@ static int shell_field(const cplx<R>* spec, cplx<R>* ws, int N, int64_t P, doubl
     return fft_pass_z_c2r<R>(ws, out, N, P, sc, store, aux, acc, st);
 }
 
+// The indicator fields I_i depend only on (N, L, k1, k2, theta): their sums (sum I_i^2 per shell,
+// sum I1 I2 I3 per theta bin) are cached per device and precision.
+struct IKey {
+    int dev, N, prec;
+    std::vector<double> v;  // BoxSize, k1, k2, theta...
+    bool operator<(const IKey& o) const {
+        if (dev != o.dev) return dev < o.dev;
+        if (N != o.N) return N < o.N;
+        if (prec != o.prec) return prec < o.prec;
+        return v < o.v;
+    }
+};
+static std::mutex g_imu;
+static std::map<IKey, std::vector<double>> g_isums;  // 2 per shell: sum I^2, sum I * (I1 I2)
+
 template <typename R>
 static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, const double* theta, int nb, int mas_p,
                   double* B, double* Q, double* kall, double* Pk, hipStream_t st) {
@@ -141,23 +156,49 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
     if (rc) return rc;
     rc = zero_fill(acc, sizeof(double) * nacc, st);
     if (rc) return rc;
+    int dev = 0;
+    PYL_HIP(hipGetDevice(&dev));
+    IKey key{dev, N, (int)sizeof(R), {BoxSize, k1, k2}};
+    key.v.insert(key.v.end(), theta, theta + nb);
+    std::vector<double> isums;
+    {
+        std::lock_guard<std::mutex> lk(g_imu);
+        auto it = g_isums.find(key);
+        if (it != g_isums.end()) isums = it->second;
+    }
+    const bool need_i = isums.empty();
     rc = fft3d_r2c<R>(delta, spec, N, P, st);
     if (rc) return rc;
-    // shell k1: d1, I1
+    // shell k1: d1 (and I1); shell k2: store d1*d2 (and I1*I2)
     if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 0, win, d1, 1, nullptr, acc + 0, st))) return rc;
-    if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 1, win, I1, 1, nullptr, acc + 2, st))) return rc;
-    // shell k2: store d1*d2 and I1*I2
     if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 0, win, d12, 2, d1, acc + 4, st))) return rc;
-    if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
+    if (need_i) {
+        if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 1, win, I1, 1, nullptr, acc + 2, st))) return rc;
+        if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
+    }
     for (int j = 0; j < nb; ++j) {
         double* a = acc + 4 * (j + 2);
         if ((rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 0, win, nullptr, 0, d12, a, st))) return rc;
-        if ((rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 1, win, nullptr, 0, I12, a + 2, st)))
+        if (need_i && (rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 1, win, nullptr, 0, I12, a + 2, st)))
             return rc;
     }
     std::vector<double> h(nacc);
     PYL_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st));
     PYL_HIP(hipStreamSynchronize(st));
+    if (need_i) {
+        isums.resize(2 * (size_t)(nb + 2));
+        for (int i = 0; i < nb + 2; ++i) {
+            isums[2 * i] = h[4 * i + 2];
+            isums[2 * i + 1] = h[4 * i + 3];
+        }
+        std::lock_guard<std::mutex> lk(g_imu);
+        g_isums[key] = isums;
+    } else {
+        for (int i = 0; i < nb + 2; ++i) {
+            h[4 * i + 2] = isums[2 * i];
+            h[4 * i + 3] = isums[2 * i + 1];
+        }
+    }
     const double unitP = pow(BoxSize / ((double)N * N), 3);
     const double unitB = pow(BoxSize * BoxSize / ((double)N * N * N), 3);
     for (int i = 0; i < nb + 2; ++i) {

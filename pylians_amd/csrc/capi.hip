@@ -2,6 +2,11 @@
 #include <math.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
 #include "../../include/pylians_hip.h"
 #include "adjoint.hpp"
 #include "common.hpp"
@@ -198,6 +203,113 @@ extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int
                                  (cplx<double>*)deltak_out, false, nullptr, 1.0, st);
 }
 
+// ------------------------------------------------------------------ deposit autotuner
+// Per (device, dims, scheme, precision, weights, floor(log2 n)) the fused pipeline times the
+// cell-binned and the direct-atomic deposit once (both overwrite the grid, so the trial runs are
+// harmless), keeps the faster and caches the choice.  pyl_tune_export / pyl_tune_import move the
+// cache as text ("dims scheme prec hasw nlog choice" lines; choice 0 = binned, 1 = atomic).
+namespace {
+struct TuneKey {
+    int dev, dims, scheme, prec, hasw, nlog;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(dev, dims, scheme, prec, hasw, nlog) < std::tie(o.dev, o.dims, o.scheme, o.prec, o.hasw, o.nlog);
+    }
+};
+std::mutex g_tmu;
+std::map<TuneKey, int> g_tune;
+bool g_tune_on = true;
+}  // namespace
+
+static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
+                         bool binned, void* grid, const double* dmass, hipStream_t st) {
+    if (binned) return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st);
+    return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | (f64 ? DEP_GRID_F64 : 0), grid, st);
+}
+
+static int tune_choice(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
+                       void* grid, const double* dmass, hipStream_t st, bool* binned) {
+    if (!binned_supported(3, dims)) {
+        *binned = false;
+        return OK;
+    }
+    int dev = 0;
+    PYL_HIP(hipGetDevice(&dev));
+    int nlog = 0;
+    while ((1LL << (nlog + 1)) <= n) ++nlog;
+    TuneKey key{dev, dims, scheme, f64 ? 8 : 4, W ? 1 : 0, nlog};
+    {
+        std::lock_guard<std::mutex> lk(g_tmu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            *binned = it->second == 0;
+            return OK;
+        }
+        if (!g_tune_on) {
+            *binned = true;
+            return OK;
+        }
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PYL_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) {   // never time inside a capture
+        *binned = true;
+        return OK;
+    }
+    hipEvent_t e[3];
+    for (auto& x : e) PYL_HIP(hipEventCreate(&x));
+    float ms[2] = {0.f, 0.f};
+    int rc = OK;
+    for (int c = 0; c < 2 && rc == OK; ++c) {
+        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st);  // warm
+        if (rc) break;
+        PYL_HIP(hipEventRecord(e[0], st));
+        for (int r = 0; r < 2 && rc == OK; ++r)
+            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st);
+        PYL_HIP(hipEventRecord(e[1], st));
+        PYL_HIP(hipEventSynchronize(e[1]));
+        PYL_HIP(hipEventElapsedTime(&ms[c], e[0], e[1]));
+    }
+    for (auto& x : e) (void)hipEventDestroy(x);
+    if (rc) return rc;
+    *binned = ms[0] <= ms[1];
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_tune[key] = *binned ? 0 : 1;
+    return OK;
+}
+
+extern "C" int pyl_tune(int enable) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_tune_on = enable != 0;
+    return OK;
+}
+
+extern "C" int pyl_tune_export(char* buf, int len) {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    std::string s;
+    for (auto& kv : g_tune) {
+        char line[128];
+        snprintf(line, sizeof(line), "%d %d %d %d %d %d\n", kv.first.dims, kv.first.scheme, kv.first.prec,
+                 kv.first.hasw, kv.first.nlog, kv.second);
+        s += line;
+    }
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", s.c_str());
+    return (int)s.size();
+}
+
+extern "C" int pyl_tune_import(const char* text) {
+    PYL_CHECK(text, E_ARG, "null text");
+    int dev = 0;
+    PYL_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tmu);
+    const char* p = text;
+    int d, sc, pr, hw, nl, ch, used;
+    while (sscanf(p, "%d %d %d %d %d %d%n", &d, &sc, &pr, &hw, &nl, &ch, &used) == 6) {
+        g_tune[TuneKey{dev, d, sc, pr, hw, nl}] = ch;
+        p += used;
+    }
+    return OK;
+}
+
 extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
                                 int axis, int prec, int flags, double* out, void* stream) {
     int rc = check_fft(dims, prec);
@@ -217,14 +329,19 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
         rc = weight_sum(W, n, dmass, st);
         if (rc) return rc;
     }
-    // binned path: the deposit writes delta = rho/rhobar - 1 itself (float64 before one rounding)
-    const bool binned = !(flags & DEP_ATOMIC) && binned_supported(3, dims);
-    if (binned) {
-        rc = deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st);
+    // binned path: the deposit writes delta = rho/rhobar - 1 itself (float64 before one rounding);
+    // atomic path: rho, normalised inside pass Z
+    bool binned;
+    if (flags & DEP_ATOMIC) {
+        binned = false;
+    } else if (flags & DEP_BINNED) {
+        PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
+        binned = true;
     } else {
-        PYL_CHECK(!(flags & DEP_BINNED), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
-        rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | (f64 ? DEP_GRID_F64 : 0), grid, st);
+        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &binned);
+        if (rc) return rc;
     }
+    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, binned, grid, dmass, st);
     if (rc) return rc;
     const int pkflags = flags & PYL_PK_ALL;
     if (f64)

@@ -50,3 +50,14 @@ def test_batch_without_graph(gpu):
     for g, j in zip(b.results(), jobs):
         o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_floor=True)
         assert_pk_close(g, o, rtol=1e-5, floor=fl)
+
+
+def test_autotuner_cache_roundtrip(gpu, tmp_path):
+    from pylians_amd import pipeline, tuning
+    pos = synth.uniform(40 ** 3, 300.0, seed=5)
+    pipeline.PkParticles(pos, 300.0, 64, MAS="TSC", axis=1)
+    ent = tuning.entries()
+    assert any(e["dims"] == 64 and e["scheme"] == 3 for e in ent)
+    path = tuning.save(str(tmp_path / "tune.json"))
+    assert tuning.load(path) == len(ent)
+    assert tuning.entries() == ent

@@ -32,3 +32,21 @@ def test_bk_parity(PKL, N, prec):
     assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= tol
     assert np.max(np.abs(g.B - o.B) / np.abs(o.B)) <= tol, (g.B, o.B)
     assert np.max(np.abs(g.Q - o.Q) / np.abs(o.Q)) <= tol
+
+
+def test_bk_cached_indicator_sums_reused(PKL):
+    """Second call with the same (N, L, k1, k2, theta) reuses the cached indicator-field sums."""
+    N, L = 32, 300.0
+    kF = 2 * np.pi / L
+    theta = np.linspace(0.2, 3.0, 4)
+    outs = []
+    for seed in (1, 2):
+        pos = synth.zeldovich(N, L, seed=seed, rms_disp=5.0)
+        rho = C.deposit_f64(pos, N, L, "CIC")
+        delta = (rho / rho.mean() - 1.0).astype(np.float32)
+        g = PKL.Bk(delta, L, 3.3 * kF, 5.2 * kF, theta, MAS="CIC")
+        o = Bo.Bk(delta.astype(np.float64), L, 3.3 * kF, 5.2 * kF, theta, MAS="CIC")
+        assert np.max(np.abs(g.B - o.B) / np.abs(o.B)) <= 1e-5
+        assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= 1e-5
+        outs.append(g.B)
+    assert not np.allclose(outs[0], outs[1])
