@@ -105,6 +105,17 @@ int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int
 int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
                      int prec, int flags, double* out, void* stream);
 
+/* Replaces Pk_library.XPk (Pk_library.pyx:534-798): auto and cross power spectra of nfields
+ * (1..8) fields.  deltas: HOST array of nfields device pointers to (N,N,N) reals of precision prec;
+ * mas_p: HOST array of window exponents.  out: HOST float64 array of pyl_xpk_layout values:
+ *   k3D[kmax] Nmodes3D[kmax] k1D[kmax_par] Nmodes1D[kmax_par] kpar[n2] kper[n2] Nmodes2D[n2]
+ *   then per pair (autos 0..F-1, then crosses (0,1),(0,2),...,(1,2),...):
+ *   P[kmax][3] (P0,P2,P4)  P1D[kmax_par]  P2D[n2].
+ * The call synchronises `stream`. */
+int pyl_xpk_layout(int dims, int nfields, int64_t* out_len);
+int pyl_xpk(const void* const* deltas, int nfields, int dims, double BoxSize, int axis, const int* mas_p, int prec,
+            double* out, void* stream);
+
 /* ---------------------------------------------------------------- bispectrum
  * Replaces Pk_library.Bk (bispectrum_library.pyx:32-196).  theta: HOST array of ntheta angles.
  * B, Q (ntheta) and k, Pk (ntheta+2) are HOST float64 arrays; the call synchronises `stream`. */

@@ -214,3 +214,83 @@ class Pk:
         self.Pk = P
         self.Pkphase = _div(s["Pkphase"][1:], N3) * unit
         self.Nmodes3D = N3
+
+
+class XPk:
+    """float64 restatement of ``Pk_library.XPk`` (Pk_library.pyx:534-798): auto and cross
+    multipoles of several fields, each with its own window (MAS list).
+
+    Attributes: k1D, Pk1D (kmax_par, F), PkX1D (kmax_par, X), Nmodes1D, kpar, kper,
+    Pk2D (n2, F), PkX2D (n2, X), Nmodes2D, k3D, Pk (kmax, 3, F), XPk (kmax, 3, X), Nmodes3D;
+    crosses ordered (0,1), (0,2), ..., (1,2), ... (Pk_library.pyx:714-728).
+    """
+
+    def __init__(self, delta, BoxSize, axis=2, MAS=None, threads=1):
+        fields = len(delta)
+        dims = np.asarray(delta[0]).shape[0]
+        for d in delta[1:]:
+            if np.asarray(d).shape[0] != dims:
+                raise SystemExit("Fields have different grid sizes!!!")
+        BoxSize = float(BoxSize)
+        kF, kN, kmax_par, kmax_per, kmax = frequencies(BoxSize, dims)
+        middle = dims // 2
+        kx, ky, kz, indep = mode_grid(dims)
+        shape = (dims, dims, middle + 1)
+        prefact = np.pi / dims
+        indep = np.broadcast_to(indep, shape)
+        KX, KY, KZ = (np.broadcast_to(v, shape) for v in (kx, ky, kz))
+        k2 = (KX * KX + KY * KY + KZ * KZ)[indep]
+        kpar_s = (KX, KY, KZ)[axis][indep]
+        perp = [v for i, v in enumerate((KX, KY, KZ)) if i != axis]
+        k_per = isqrt_array(perp[0] * perp[0] + perp[1] * perp[1])[indep]
+        k = np.sqrt(k2.astype(np.float64))
+        kidx = isqrt_array(k2)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            mu = np.where(k == 0, 0.0, kpar_s / k)
+        mu2 = mu * mu
+        L2 = (3.0 * mu2 - 1.0) / 2.0
+        L4 = (35.0 * mu2 * mu2 - 30.0 * mu2 + 3.0) / 8.0
+        k_par = np.abs(kpar_s)
+        m1 = k2 <= middle * middle
+        i2 = (kmax_par + 1) * k_per + k_par
+        n1, n2, n3 = kmax_par + 1, (kmax_par + 1) * (kmax_per + 1), kmax + 1
+        dks = []
+        for f in range(fields):
+            p = MAS_function(MAS[f])
+            corr = (MAS_correction(prefact * kx, p) * MAS_correction(prefact * ky, p) * MAS_correction(prefact * kz, p))
+            dks.append((FFT3Dr(delta[f]) * np.broadcast_to(corr, shape))[indep])
+        pairs = [(i, i) for i in range(fields)] + [(i, j) for i in range(fields) for j in range(i + 1, fields)]
+        sums = []
+        for (a, b) in pairs:
+            v = (dks[a] * np.conj(dks[b])).real
+            s3 = np.stack([np.bincount(kidx, weights=v * w, minlength=n3) for w in (1.0, L2, L4)], axis=1)
+            s1 = np.bincount(k_par[m1], weights=v[m1], minlength=n1)
+            s2 = np.bincount(i2, weights=v, minlength=n2)
+            sums.append((s3, s1, s2))
+        N1 = np.bincount(k_par[m1], minlength=n1).astype(np.float64)
+        N2 = np.bincount(i2, minlength=n2).astype(np.float64)
+        N3 = np.bincount(kidx, minlength=n3).astype(np.float64)
+        k1 = np.bincount(k_par[m1], weights=k_par[m1].astype(np.float64), minlength=n1)
+        k3 = np.bincount(kidx, weights=k, minlength=n3)
+        fact = (BoxSize / dims ** 2) ** 3
+        # 1-D (Pk_library.pyx:734-748)
+        self.k1D = _div(k1[1:], N1[1:]) * kF
+        kmaxper = np.sqrt(kN ** 2 - self.k1D ** 2)
+        area = _div(np.pi * kmaxper ** 2, N1[1:]) / (2.0 * np.pi) ** 2
+        P1 = np.stack([s[1][1:] * fact * area for s in sums], axis=1)
+        self.Nmodes1D = N1[1:]
+        self.Pk1D, self.PkX1D = P1[:, :fields], P1[:, fields:]
+        # 2-D (Pk_library.pyx:751-766)
+        kp = np.arange(kmax_par + 1)
+        kq = np.arange(kmax_per + 1)
+        self.kpar = np.tile(0.5 * (2 * kp + 1) * kF, kmax_per + 1)
+        self.kper = np.repeat(0.5 * (2 * kq + 1) * kF, kmax_par + 1)
+        P2 = np.stack([_div(s[2] * fact, N2) for s in sums], axis=1)
+        self.Nmodes2D = N2
+        self.Pk2D, self.PkX2D = P2[:, :fields], P2[:, fields:]
+        # 3-D (Pk_library.pyx:768-785)
+        check_number_modes(N3, dims)
+        self.k3D = _div(k3[1:], N3[1:]) * kF
+        P3 = np.stack([_div(s[0][1:] * np.array([1.0, 5.0, 9.0]), N3[1:, None]) * fact for s in sums], axis=2)
+        self.Nmodes3D = N3[1:]
+        self.Pk, self.XPk = P3[:, :, :fields], P3[:, :, fields:]

@@ -28,7 +28,7 @@ from . import _lib
 VERBOSE = False
 
 __all__ = ["frequencies", "frequencies_2D", "MAS_function", "MAS_correction", "check_number_modes",
-           "check_number_modes_2D", "FFT3Dr_f", "FFT3Dr_d", "IFFT3Dr_f", "IFFT3Dr_d", "Pk", "Bk",
+           "check_number_modes_2D", "FFT3Dr_f", "FFT3Dr_d", "IFFT3Dr_f", "IFFT3Dr_d", "Pk", "XPk", "Bk",
            "pk_layout"]
 
 
@@ -189,6 +189,62 @@ class Pk:
             self.delta_k = _ret(delta, dk)
         if VERBOSE:
             print("Time taken = %.2f seconds" % (time.time() - start))
+
+
+class XPk:
+    """Auto and cross power spectra of several fields (Pk_library.pyx:534-798), on the GPU.
+
+    Attributes as the reference: k1D, Pk1D (kmax_par, F), PkX1D (kmax_par, X), Nmodes1D, kpar, kper,
+    Pk2D (n2, F), PkX2D (n2, X), Nmodes2D, k3D, Pk (kmax, 3, F), XPk (kmax, 3, X), Nmodes3D with
+    X = F(F-1)/2 crosses ordered (0,1), (0,2), ..., (1,2), ...
+    """
+
+    def __init__(self, delta, BoxSize, axis=2, MAS=None, threads=1):
+        import ctypes
+        fields = len(delta)
+        dims = int(delta[0].shape[0])
+        for d in delta[1:]:
+            if int(d.shape[0]) != dims:
+                print("Fields have different grid sizes!!!")
+                sys.exit()
+        f64 = all(((x.dtype == torch.float64) if isinstance(x, torch.Tensor) else (np.asarray(x).dtype == np.float64))
+                  for x in delta)
+        rt = torch.float64 if f64 else torch.float32
+        devs = [_grid3(x, rt, "delta") for x in delta]
+        ptrs = (ctypes.c_void_p * fields)(*[t.data_ptr() for t in devs])
+        mas = (ctypes.c_int * fields)(*[MAS_function(m) for m in MAS])
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().pyl_xpk_layout(dims, fields, ctypes.byref(n)))
+        out = np.zeros(n.value)
+        rc = _lib.lib().pyl_xpk(ptrs, fields, dims, float(BoxSize), int(axis), mas, _lib.F64 if f64 else _lib.F32,
+                                out.ctypes.data_as(ctypes.c_void_p), _lib.stream())
+        _lib.check(rc)
+        kmax_par, kmax_per, kmax, _ = pk_layout(dims)
+        n2 = (kmax_par + 1) * (kmax_per + 1)
+        o = 0
+
+        def take(m):
+            nonlocal o
+            v = out[o:o + m]
+            o += m
+            return v
+
+        self.k3D, self.Nmodes3D = take(kmax), take(kmax)
+        self.k1D, self.Nmodes1D = take(kmax_par), take(kmax_par)
+        self.kpar, self.kper, self.Nmodes2D = take(n2), take(n2), take(n2)
+        C = fields * (fields + 1) // 2
+        P3, P1, P2 = [], [], []
+        for _ in range(C):
+            P3.append(take(3 * kmax).reshape(kmax, 3))
+            P1.append(take(kmax_par))
+            P2.append(take(n2))
+        P3 = np.stack(P3, axis=2)
+        P1 = np.stack(P1, axis=1)
+        P2 = np.stack(P2, axis=1)
+        self.Pk, self.XPk = P3[:, :, :fields], P3[:, :, fields:]
+        self.Pk1D, self.PkX1D = P1[:, :fields], P1[:, fields:]
+        self.Pk2D, self.PkX2D = P2[:, :fields], P2[:, fields:]
+        check_number_modes(np.concatenate([[1.0], self.Nmodes3D]), dims)
 
 
 class Bk:

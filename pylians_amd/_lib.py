@@ -45,6 +45,8 @@ SIGNATURES = {
     "pyl_graph_create": [_vp, _i, _vp, _vp],
     "pyl_graph_launch": [_vp, _vp],
     "pyl_graph_destroy": [_vp],
+    "pyl_xpk_layout": [_i, _i, _vp],
+    "pyl_xpk": [_vp, _i, _i, _d, _i, _vp, _i, _vp, _vp],
     "pyl_tune": [_i],
     "pyl_tune_export": [_c.c_char_p, _i],
     "pyl_tune_import": [_c.c_char_p],

@@ -1,0 +1,324 @@
+// Auto and cross power spectra of several fields (Pk_library.XPk, Pk_library.pyx:534-798) on gfx950.
+//
+// Every field is transformed (r2c, stored half spectrum); one binning kernel then walks the
+// independent modes once per group of (field a, field b) pairs, applies each field's window
+// (its own MAS, Pk_library.pyx:688-697) and accumulates Re(d_a conj(d_b)) into the 3-D
+// multipoles, 1-D and 2-D histograms (Pk_library.pyx:699-728) in float64.  The block/column
+// assignment is the one of the fused Pk pass: lines run along y (line of sight x) or x, and each
+// persistent block owns one line coordinate u (k_par = |k_u|) or one kz block (line of sight z),
+// so its 2-D window stays in LDS.
+#include <math.h>
+
+#include <vector>
+
+#include "../../include/pylians_hip.h"
+#include "fft.hpp"
+#include "pk.hpp"
+#include "util.hpp"
+
+namespace pyl {
+
+const double* pk_window_table(int N, int p);
+
+constexpr int XMAXF = 8;   // fields per call
+constexpr int XMAXC = 4;   // pairs per binning launch
+
+struct XBinArgs {
+    const void* spec[XMAXF];
+    const double* win[XMAXF];
+    int ca[XMAXC], cb[XMAXC];
+    int nc;
+    long long P;
+    int N, axis, last, fixk, nchunk, kmax_par, kmax_per, n1, n3, n2;
+    double* s3;  // [nc][n3][3]
+    double* s1;  // [nc][n1]
+    double* s2;  // [nc][n2]
+};
+
+template <typename R, int NL, int T>
+__global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
+    constexpr int TPC = T / NL;
+    extern __shared__ double hs[];
+    const int N = a.N, nc = a.nc, nper = a.kmax_per + 1;
+    const int wlen = (a.fixk ? NL : 1) * nper;
+    double* h3 = hs;                       // [n3][nc][3]
+    double* h1 = h3 + a.n3 * nc * 3;       // [nc][n1]
+    double* h2 = h1 + nc * a.n1;           // [nc][wlen]
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.n3 * nc * 3 + nc * a.n1 + nc * wlen; i += T) hs[i] = 0.0;
+    __syncthreads();
+    const int middle = N / 2;
+    const bool even = (N % 2) == 0;
+    const long long P = a.P;
+    const int nkzb = (int)((P + NL - 1) / NL);
+    const long long EST = a.last == 0 ? (long long)N * P : P;
+    const long long UST = a.last == 0 ? P : (long long)N * P;
+    const int l = tid % NL, jrow = tid / NL;
+    const int E = (N + TPC - 1) / TPC;
+    int u0, u1, kzb0, kzb1;
+    if (a.fixk) {
+        const int kzb = blockIdx.x / a.nchunk, ch = blockIdx.x % a.nchunk;
+        const int U = (N + a.nchunk - 1) / a.nchunk;
+        u0 = ch * U;
+        u1 = min(N, u0 + U);
+        kzb0 = kzb;
+        kzb1 = kzb + 1;
+    } else {
+        u0 = blockIdx.x;
+        u1 = u0 + 1;
+        kzb0 = 0;
+        kzb1 = nkzb;
+    }
+    const int kpar_u = a.fixk ? 0 : abs(kfreq(u0, N));
+    for (int u = u0; u < u1; ++u)
+        for (int kzb = kzb0; kzb < kzb1; ++kzb) {
+            const int kz = kzb * NL + l;
+            if (kz >= P) continue;
+            const long long base = (long long)u * UST + kz;
+            const int ku = kfreq(u, N);
+            const bool zplane = (kz == 0) || (kz == middle && even);
+            int cur3 = -1, cur2 = -1;
+            double acc[XMAXC][3], b1[XMAXC], b2[XMAXC];
+            for (int c = 0; c < XMAXC; ++c) {
+                acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
+                b1[c] = b2[c] = 0.0;
+            }
+            for (int q = 0; q < E; ++q) {
+                const int j = jrow * E + q;
+                if (j >= N) break;
+                const int kj = kfreq(j, N);
+                const int kx = a.last == 0 ? kj : ku;
+                const int ky = a.last == 0 ? ku : kj;
+                if (zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0))) continue;
+                const int xx = a.last == 0 ? j : u, yy = a.last == 0 ? u : j;
+                const int k2 = kx * kx + ky * ky + kz * kz;
+                const int bin = isqrt_i(k2);
+                const int kps = a.axis == 0 ? kx : (a.axis == 1 ? ky : kz);
+                const R mu2 = k2 == 0 ? (R)0 : (R)(kps * kps) / (R)k2;
+                const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
+                const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
+                R val[XMAXC];
+#pragma unroll
+                for (int c = 0; c < XMAXC; ++c) {
+                    if (c >= nc) break;
+                    const int fa = a.ca[c], fb = a.cb[c];
+                    const double* wa = a.win[fa];
+                    const double* wb = a.win[fb];
+                    const cplx<R> va = scale(((const cplx<R>*)a.spec[fa])[base + j * EST],
+                                             (R)(wa[xx] * wa[yy] * wa[kz]));
+                    const cplx<R> vb = fa == fb ? va : scale(((const cplx<R>*)a.spec[fb])[base + j * EST],
+                                                             (R)(wb[xx] * wb[yy] * wb[kz]));
+                    val[c] = va.x * vb.x + va.y * vb.y;
+                }
+                if (bin != cur3) {
+                    if (cur3 >= 0)
+                        for (int c = 0; c < nc; ++c)
+                            for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
+                    cur3 = bin;
+                    for (int c = 0; c < XMAXC; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
+                }
+                for (int c = 0; c < nc; ++c) {
+                    acc[c][0] += (double)val[c];
+                    acc[c][1] += (double)(val[c] * L2);
+                    acc[c][2] += (double)(val[c] * L4);
+                }
+                if (k2 <= middle * middle)
+                    for (int c = 0; c < nc; ++c) b1[c] += (double)val[c];
+                const int kper = isqrt_i(a.axis == 0 ? ky * ky + kz * kz
+                                                     : (a.axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
+                const int w2 = (a.fixk ? l * nper : 0) + kper;
+                if (w2 != cur2) {
+                    if (cur2 >= 0)
+                        for (int c = 0; c < nc; ++c) atomicAdd(&h2[c * wlen + cur2], b2[c]);
+                    cur2 = w2;
+                    for (int c = 0; c < XMAXC; ++c) b2[c] = 0.0;
+                }
+                for (int c = 0; c < nc; ++c) b2[c] += (double)val[c];
+            }
+            if (cur3 >= 0)
+                for (int c = 0; c < nc; ++c)
+                    for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
+            if (cur2 >= 0)
+                for (int c = 0; c < nc; ++c) atomicAdd(&h2[c * wlen + cur2], b2[c]);
+            const int kpc = a.fixk ? kz : kpar_u;
+            for (int c = 0; c < nc; ++c)
+                if (b1[c] != 0.0) atomicAdd(&h1[c * a.n1 + kpc], b1[c]);
+        }
+    __syncthreads();
+    for (int i = tid; i < a.n3 * nc * 3; i += T) {
+        if (h3[i] == 0.0) continue;
+        const int bin = i / (nc * 3), c = (i / 3) % nc, m = i % 3;
+        atomicAdd(&a.s3[((long long)c * a.n3 + bin) * 3 + m], h3[i]);
+    }
+    for (int i = tid; i < nc * a.n1; i += T)
+        if (h1[i] != 0.0) atomicAdd(&a.s1[i], h1[i]);
+    for (int i = tid; i < nc * wlen; i += T) {
+        if (h2[i] == 0.0) continue;
+        const int c = i / wlen, r = i % wlen, kper = r % nper;
+        const int kpar = a.fixk ? kzb0 * NL + r / nper : kpar_u;
+        if (kpar <= a.kmax_par) atomicAdd(&a.s2[(long long)c * a.n2 + (long long)(a.kmax_par + 1) * kper + kpar], h2[i]);
+    }
+}
+
+static int num_cus_x() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    return n;
+}
+
+template <typename R>
+static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int axis, const int* mas_p, double* out,
+                   hipStream_t st) {
+    constexpr int NL = 16, T = 256;
+    int rc = OK;
+    const int64_t P = half_len(N);
+    const PkGeometry* g = pk_geometry(N, axis, &rc);
+    if (rc) return rc;
+    const PkDims& d = g->d;
+    const size_t nspec = (size_t)N * N * P;
+    cplx<R>* spec = (cplx<R>*)workspace(WS_SPEC, sizeof(cplx<R>) * nspec * F, &rc);
+    if (rc) return rc;
+    for (int f = 0; f < F; ++f) {
+        rc = fft3d_r2c<R>((const R*)deltas[f], spec + nspec * f, N, P, st);
+        if (rc) return rc;
+    }
+    std::vector<int> pa, pb;
+    for (int i = 0; i < F; ++i) {
+        pa.push_back(i);
+        pb.push_back(i);
+    }
+    for (int i = 0; i < F; ++i)
+        for (int j = i + 1; j < F; ++j) {
+            pa.push_back(i);
+            pb.push_back(j);
+        }
+    const int C = (int)pa.size();
+    const int per = 3 * d.n3() + d.n1() + d.n2();
+    double* sums = (double*)workspace(WS_SUMS, sizeof(double) * (size_t)per * C, &rc);
+    if (rc) return rc;
+    rc = zero_fill(sums, sizeof(double) * (size_t)per * C, st);
+    if (rc) return rc;
+    XBinArgs a{};
+    for (int f = 0; f < F; ++f) {
+        a.spec[f] = spec + nspec * f;
+        a.win[f] = pk_window_table(N, mas_p[f]);
+        PYL_CHECK(a.win[f], E_ALLOC, "window table allocation failed");
+    }
+    a.P = P;
+    a.N = N;
+    a.axis = axis;
+    a.last = pk_last_axis(axis);
+    a.fixk = axis == 2 ? 1 : 0;
+    a.kmax_par = d.kmax_par;
+    a.kmax_per = d.kmax_per;
+    a.n1 = d.n1();
+    a.n2 = d.n2();
+    a.n3 = d.n3();
+    const int nper = d.kmax_per + 1;
+    const int wlen = (a.fixk ? NL : 1) * nper;
+    // pairs per launch: as many (<= 4) as fit the LDS histograms
+    int cmax = XMAXC;
+    while (cmax > 1 && (size_t)cmax * (3 * d.n3() + d.n1() + wlen) * 8 > 120 * 1024) --cmax;
+    const int nkzb = (int)((P + NL - 1) / NL);
+    a.nchunk = 1;
+    int grid = N;
+    if (a.fixk) {
+        a.nchunk = (num_cus_x() * 4 + nkzb - 1) / nkzb;
+        if (a.nchunk > N) a.nchunk = N;
+        grid = nkzb * a.nchunk;
+    }
+    for (int c0 = 0; c0 < C; c0 += cmax) {
+        a.nc = C - c0 < cmax ? C - c0 : cmax;
+        for (int c = 0; c < a.nc; ++c) {
+            a.ca[c] = pa[c0 + c];
+            a.cb[c] = pb[c0 + c];
+        }
+        // a launch group of nc pairs owns nc * per doubles, kind-major: [nc][n3][3] | [nc][n1] | [nc][n2]
+        double* base = sums + (size_t)per * c0;
+        a.s3 = base;
+        a.s1 = base + (size_t)3 * d.n3() * a.nc;
+        a.s2 = a.s1 + (size_t)d.n1() * a.nc;
+        const size_t shm = sizeof(double) * (size_t)a.nc * (3 * d.n3() + d.n1() + wlen);
+        hipLaunchKernelGGL((k_xbin<R, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, a);
+        PYL_HIP(hipGetLastError());
+    }
+    std::vector<double> h((size_t)per * C);
+    PYL_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    PYL_HIP(hipStreamSynchronize(st));
+    // finalize (Pk_library.pyx:730-785)
+    const double kF = 2.0 * M_PI / BoxSize, kN = d.middle * kF;
+    const double u1 = BoxSize / ((double)N * N);
+    const double fact = u1 * u1 * u1;
+    const int kmax = d.kmax, kp = d.kmax_par, n2 = d.n2();
+    double* o = out;
+    for (int i = 0; i < kmax; ++i) o[i] = g->h_k3sum[i + 1] / (double)g->h_n3[i + 1] * kF;
+    o += kmax;
+    for (int i = 0; i < kmax; ++i) o[i] = (double)g->h_n3[i + 1];
+    o += kmax;
+    for (int i = 0; i < kp; ++i) o[i] = ((double)(i + 1) * g->h_n1[i + 1] / (double)g->h_n1[i + 1]) * kF;
+    double* k1D = o;
+    o += kp;
+    for (int i = 0; i < kp; ++i) o[i] = (double)g->h_n1[i + 1];
+    o += kp;
+    for (int i = 0; i < n2; ++i) o[i] = 0.5 * (2 * (i % (kp + 1)) + 1) * kF;
+    o += n2;
+    for (int i = 0; i < n2; ++i) o[i] = 0.5 * (2 * (i / (kp + 1)) + 1) * kF;
+    o += n2;
+    for (int i = 0; i < n2; ++i) o[i] = (double)g->h_n2[i];
+    o += n2;
+    for (int c = 0; c < C; ++c) {
+        // locate this pair's sums inside its launch group
+        const int c0 = (c / cmax) * cmax, nc = (C - c0 < cmax ? C - c0 : cmax), cc = c - c0;
+        const double* gb = h.data() + (size_t)per * c0;
+        const double* s3 = gb + ((size_t)cc * d.n3()) * 3;
+        const double* s1 = gb + (size_t)3 * d.n3() * nc + (size_t)cc * d.n1();
+        const double* s2 = gb + (size_t)3 * d.n3() * nc + (size_t)d.n1() * nc + (size_t)cc * n2;
+        for (int i = 0; i < kmax; ++i) {
+            const double n = (double)g->h_n3[i + 1];
+            o[3 * i + 0] = (s3[3 * (i + 1) + 0] / n) * fact;
+            o[3 * i + 1] = (s3[3 * (i + 1) + 1] * 5.0 / n) * fact;
+            o[3 * i + 2] = (s3[3 * (i + 1) + 2] * 9.0 / n) * fact;
+        }
+        o += 3 * kmax;
+        for (int i = 0; i < kp; ++i) {
+            const double n = (double)g->h_n1[i + 1];
+            const double kmaxper = sqrt(kN * kN - k1D[i] * k1D[i]);
+            o[i] = s1[i + 1] * fact * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
+        }
+        o += kp;
+        for (int i = 0; i < n2; ++i) o[i] = s2[i] * fact / (double)g->h_n2[i];
+        o += n2;
+    }
+    return OK;
+}
+
+}  // namespace pyl
+
+using namespace pyl;
+
+extern "C" int pyl_xpk_layout(int dims, int nfields, int64_t* out_len) {
+    PYL_CHECK(dims > 1 && nfields >= 1 && nfields <= XMAXF, E_ARG, "bad dims / nfields (1..%d)", XMAXF);
+    const PkDims d = pk_dims(dims, half_len(dims));
+    const int64_t C = (int64_t)nfields * (nfields + 1) / 2;
+    if (out_len)
+        *out_len = 2 * (int64_t)d.kmax + 2 * (int64_t)d.kmax_par + 3 * (int64_t)d.n2() +
+                   C * (3 * (int64_t)d.kmax + d.kmax_par + d.n2());
+    return OK;
+}
+
+extern "C" int pyl_xpk(const void* const* deltas, int nfields, int dims, double BoxSize, int axis, const int* mas_p,
+                       int prec, double* out, void* stream) {
+    PYL_CHECK(prec == PYL_F32 || prec == PYL_F64, E_ARG, "prec must be 4 or 8");
+    PYL_CHECK(fft_size_supported(dims), E_UNSUPPORTED, "grid size %d not supported", dims);
+    PYL_CHECK(nfields >= 1 && nfields <= XMAXF, E_ARG, "nfields must be in [1, %d]", XMAXF);
+    PYL_CHECK(axis >= 0 && axis <= 2 && BoxSize > 0 && deltas && mas_p && out, E_ARG, "bad arguments");
+    for (int f = 0; f < nfields; ++f) {
+        PYL_CHECK(deltas[f], E_ARG, "null field %d", f);
+        PYL_CHECK(mas_p[f] >= 0 && mas_p[f] <= 4, E_ARG, "window exponent must be in [0, 4]");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == PYL_F32) return xpk_run<float>(deltas, nfields, dims, BoxSize, axis, mas_p, out, st);
+    return xpk_run<double>(deltas, nfields, dims, BoxSize, axis, mas_p, out, st);
+}

@@ -79,3 +79,27 @@ def test_plane_wave_closed_form(PKL):
     assert p.Pk[b, 1] == pytest.approx(5 * p.Pk[b, 0], rel=1e-5)
     assert p.Pk[b, 2] == pytest.approx(9 * p.Pk[b, 0], rel=1e-5)
     assert int(p.Nmodes3D.sum()) + 1 == (N ** 3 - 8) // 2 + 8
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_xpk_parity(PKL, axis):
+    from oracle import pk as P
+    N, L = 32, 300.0
+    d1 = density_field(N, L, seed=11, MAS="CIC")
+    d2 = density_field(N, L, seed=12, MAS="TSC")
+    d3 = (0.5 * d1 + d2).astype(np.float32)
+    g = PKL.XPk([d1, d2, d3], L, axis=axis, MAS=["CIC", "TSC", "PCS"])
+    o = P.XPk([d1.astype(np.float64), d2.astype(np.float64), d3.astype(np.float64)], L, axis=axis,
+              MAS=["CIC", "TSC", "PCS"])
+    for name in ("Nmodes3D", "Nmodes1D", "Nmodes2D"):
+        assert np.array_equal(getattr(g, name), getattr(o, name)), name
+    assert np.allclose(g.k3D, o.k3D, rtol=1e-12) and np.allclose(g.k1D, o.k1D, rtol=1e-13)
+    for c in range(3):
+        P0 = np.abs(o.Pk[:, 0, c])
+        assert np.max(np.abs(g.Pk[:, :, c] - o.Pk[:, :, c]) / P0[:, None]) <= 1e-5, ("auto", c)
+        assert np.max(np.abs(g.Pk1D[:, c] - o.Pk1D[:, c]) / np.maximum(np.abs(o.Pk1D[:, c]), 1e-30)) <= 1e-5
+    for x, (a, b) in enumerate([(0, 1), (0, 2), (1, 2)]):
+        scale = np.sqrt(np.abs(o.Pk[:, 0, a] * o.Pk[:, 0, b]))
+        assert np.max(np.abs(g.XPk[:, :, x] - o.XPk[:, :, x]) / scale[:, None]) <= 1e-5, ("cross", x)
+        s2 = np.sqrt(np.abs(o.Pk2D[:, a] * o.Pk2D[:, b]))[1:]
+        assert np.max(np.abs(g.PkX2D[1:, x] - o.PkX2D[1:, x]) / s2) <= 1e-5

@@ -152,3 +152,27 @@ def test_c_mode_loop_matches_numpy(axis):
         assert np.allclose(a[k], b[k], rtol=1e-12, atol=1e-12), k
     c = C.pk_sums_c(dk, N, axis, "TSC", faithful_f32=True)
     assert np.allclose(c["P0"], a["P0"], rtol=1e-5)
+
+
+def test_xpk_oracle_consistency():
+    """XPk autos equal Pk of each field (pinned above); the cross of a field with a scaled copy of
+    itself is the scaled auto; cross of independent fields is symmetric in the pair order."""
+    N, L = 16, 200.0
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal((N, N, N))
+    b = rng.standard_normal((N, N, N))
+    x = P.XPk([a, 2.0 * a, b], L, axis=1, MAS=["CIC", "CIC", "TSC"])
+    pa = P.Pk(a, L, axis=1, MAS="CIC")
+    pb = P.Pk(b, L, axis=1, MAS="TSC")
+    assert np.allclose(x.Pk[:, :, 0], pa.Pk, rtol=1e-12)
+    assert np.allclose(x.Pk[:, :, 1], 4 * pa.Pk, rtol=1e-12)
+    assert np.allclose(x.Pk[:, :, 2], pb.Pk, rtol=1e-12)
+    assert np.allclose(x.XPk[:, :, 0], 2 * pa.Pk, rtol=1e-12)          # (0, 1)
+    assert np.allclose(x.Pk1D[:, 0], pa.Pk1D) and np.allclose(x.Pk2D[:, 2], pb.Pk2D)
+    assert np.allclose(x.PkX2D[:, 0], 2 * pa.Pk2D)
+    assert np.array_equal(x.Nmodes3D, pa.Nmodes3D) and np.allclose(x.k3D, pa.k3D)
+    y = P.XPk([b, a], L, axis=1, MAS=["TSC", "CIC"])
+    assert np.allclose(y.XPk[:, :, 0], x.XPk[:, :, 1], rtol=1e-12)      # (b,a) == (a,b): pair (0,2) of x
+    assert np.allclose(x.XPk[:, :, 2], 2 * y.XPk[:, :, 0], rtol=1e-12)  # (2a, b)
+    with pytest.raises(SystemExit):
+        P.XPk([a, np.zeros((8, 8, 8))], L, MAS=["CIC", "CIC"])
