@@ -24,18 +24,32 @@ METRIC = "Mparticles/s CIC+Pk end-to-end at 512^3; FP32 vs FP64 speedup at 1e-5 
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n_p, N):
-    """Per-stage compulsory bytes (input read once, materialised output written once)."""
+def algorithmic_bytes(n_p, N, S=2, tile=16):
+    """Per-kernel bytes each launch must move: inputs read once, outputs written once.
+
+    Float32 path, no weights.  The cell-binned deposit writes 12-byte particle records into tile
+    buckets (k_bin), reads them back once and writes the K^3 interior cells of each 16^3 tile plus
+    float64 partials of its BSZ boundary cells (k_tile), and the merge reads those partials and
+    writes the SHELL cells each tile owns (k_merge); K = 16 - (S - 1).
+    """
     Ng = N ** 3
     Nh = N * N * (N // 2 + 1)
+    H = S - 1
+    M, K = tile + H, tile - H
+    bsz = 2 * H * M * M + K * 2 * H * M + K * K * 2 * H
+    shell = tile ** 3 - K ** 3
+    nt = (N // tile) ** 3
     return {
-        "deposit_atomic": 12 * n_p + 4 * Ng,     # read positions, write the float32 grid
-        "deposit_binned": 12 * n_p + 4 * Ng,
+        "deposit_atomic": 12 * n_p + 4 * Ng,     # read positions, add into the float32 grid
+        "deposit_bin": 24 * n_p,                 # read positions, write bucket records
+        "deposit_tiles": 12 * n_p + nt * (4 * K ** 3 + 8 * bsz),
+        "deposit_merge": nt * (8 * bsz + 4 * shell),
         "grid_zero": 4 * Ng,
         "fft_z_r2c": 4 * Ng + 8 * Nh,            # read the real grid, write the half spectrum
-        "fft_y": 16 * Nh,                         # read + write the half spectrum
-        "fft_x_bin": 8 * Nh,                      # read the half spectrum (histograms negligible)
-        "e2e": (12 * n_p + 4 * Ng) + (4 * Ng + 8 * Nh) + 8 * Nh,
+        "fft_mid": 16 * Nh,                       # read + write the half spectrum
+        "fft_last_bin": 8 * Nh,                   # read the half spectrum (histograms negligible)
+        # whole job: positions in, overdensity grid out, grid in, spectrum out/in twice, spectrum in
+        "e2e": (12 * n_p + 4 * Ng) + (4 * Ng + 8 * Nh) + 16 * Nh + 8 * Nh,
     }
 
 
@@ -254,7 +268,7 @@ def main():
     print("# fp64: %.3f ms/step" % ms64, file=sys.stderr, flush=True)
 
     # ---- roofline of the dominant kernel
-    ab = algorithmic_bytes(n_p, N)
+    ab = algorithmic_bytes(n_p, N, S=scheme)
     per = {k: (v[1] / v[0]) for k, v in stages.items() if v[0] > 0}
     dom = max((k for k in per if k in ab), key=lambda k: per[k])
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9

@@ -137,30 +137,83 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
 
 // Column FFT over a half spectrum.  Column c (of ncols) starts at (c / CW) * CS + c % CW and
 // its N elements are EST apart.  Pass Y: CW = P, CS = N*P, EST = P.  Pass X: CW = ncols, EST = N*P.
+// Persistent blocks walk column blocks of NL columns; while one block is transformed the first-stage
+// inputs of the next one are already in flight into registers, and the last stage stores straight
+// from registers.  With a single radix stage the transform never touches LDS.
 template <typename R, int N, int NL, int T, bool INV>
 __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, long long ncols,
                                              long long CW, long long CS, long long EST,
                                              const cplx<R>* __restrict__ twg, R sc) {
     static_assert(T % NL == 0, "T must be a multiple of NL");
+    using F = First<N, NL, T>;
+    // prefetching doubles the first stage's registers; beyond 512 points it would spill
+    constexpr bool PREFETCH = N <= 512;
     __shared__ cplx<R> s[N * NL];
     __shared__ cplx<R> tw[N];
     const int tid = threadIdx.x;
     for (int i = tid; i < N; i += T) tw[i] = twg[i];
-    const int l = tid % NL;
-    const long long c = (long long)blockIdx.x * NL + l;
-    const bool valid = c < ncols;
-    const long long base = valid ? (c / CW) * CS + (c % CW) : 0;
-    for (int j = tid / NL; j < N; j += T / NL) {
-        cplx<R> v = valid ? src[base + j * EST] : mk<R>(0, 0);
-        s[j * NL + l] = INV ? conjg(v) : v;
+    const int l = tid % NL;   // every butterfly of this thread belongs to column l of the block
+    const long long nblk = (ncols + NL - 1) / NL;
+    auto col_base = [&](long long cb, bool& valid) {
+        const long long c = cb * NL + l;
+        valid = c < ncols;
+        return valid ? (c / CW) * CS + (c % CW) : 0;
+    };
+    cplx<R> nxt[F::B][F::RAD];
+    long long cb = blockIdx.x;
+    if (PREFETCH && cb < nblk) {
+        bool v;
+        const long long b = col_base(cb, v);
+        first_load<R, N, NL, T>(nxt, [&](int j, int) {
+            const cplx<R> x = v ? src[b + j * EST] : mk<R>(0, 0);
+            return INV ? conjg(x) : x;
+        }, tid);
     }
-    __syncthreads();
-    lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
-    if (valid) {
-        for (int j = tid / NL; j < N; j += T / NL) {
-            cplx<R> v = s[j * NL + l];
-            if (INV) v = conjg(v);
-            dst[base + j * EST] = scale(v, sc);
+    for (; cb < nblk; cb += gridDim.x) {
+        cplx<R> cur[F::B][F::RAD];
+        if (PREFETCH) {
+#pragma unroll
+            for (int b = 0; b < F::B; ++b)
+#pragma unroll
+                for (int m = 0; m < F::RAD; ++m) cur[b][m] = nxt[b][m];
+        } else {
+            bool v;
+            const long long b = col_base(cb, v);
+            first_load<R, N, NL, T>(cur, [&](int j, int) {
+                const cplx<R> x = v ? src[b + j * EST] : mk<R>(0, 0);
+                return INV ? conjg(x) : x;
+            }, tid);
+        }
+        const long long cbn = cb + gridDim.x;
+        if (PREFETCH && cbn < nblk) {
+            bool v;
+            const long long b = col_base(cbn, v);
+            first_load<R, N, NL, T>(nxt, [&](int j, int) {
+                const cplx<R> x = v ? src[b + j * EST] : mk<R>(0, 0);
+                return INV ? conjg(x) : x;
+            }, tid);
+        }
+        bool valid;
+        const long long base = col_base(cb, valid);
+        auto store = [&](int k, int, cplx<R> x) {
+            if (valid) dst[base + k * EST] = scale(INV ? conjg(x) : x, sc);
+        };
+        if constexpr (F::NST == 1) {
+#pragma unroll
+            for (int b = 0; b < F::B; ++b) {
+                const int g = tid + b * T;
+                if (F::NBF % T == 0 || g < F::NBF) {
+                    const int j = g / NL;
+                    Dft<R, F::RAD>::run(cur[b]);
+#pragma unroll
+                    for (int m = 0; m < F::RAD; ++m) store(j * F::RAD + m, l, cur[b][m]);
+                }
+            }
+        } else {
+            __syncthreads();  // LDS of the previous block consumed (and twiddles loaded)
+            first_stage_regs<R, N, NL, NL, T>(s, cur, tid);
+            mid_stages<R, N, NL, NL, T, 1, 1, First<N, NL, T>::RAD>(s, tw, tid);
+            last_stage_store<R, N, NL, NL, T, 1>(s, tw, store, tid);
         }
     }
 }
@@ -179,6 +232,19 @@ bool fft_size_supported(int N) {
         default:
             return false;
     }
+}
+
+int cu_count() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n = v;
+        else
+            n = 256;
+    }
+    return n;
 }
 
 template <typename R> const cplx<R>* twiddles(int N) {
@@ -238,7 +304,8 @@ static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P
 template <typename R, int N>
 static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st) {
     constexpr int NL = Geo<R>::NL;
-    constexpr int T = threads_for(N, NL);
+    // the prefetch doubles the first stage's registers: keep it at 32 (f32) / 16 (f64) elements
+    constexpr int T = clamp_i(N * NL / (sizeof(R) == 4 ? 32 : 16), 64, 512);
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
     const long long ncols = (long long)N * P;
@@ -252,7 +319,14 @@ static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bo
         CS = 0;
         EST = (long long)N * P;
     }
-    dim3 grid((unsigned)((ncols + NL - 1) / NL));
+    const long long nblk = (ncols + NL - 1) / NL;
+    // persistent grid: about as many blocks as fit on the chip at once (LDS bound)
+    constexpr int lds = (N * NL + N) * (int)sizeof(cplx<R>);
+    int per_cu = (160 * 1024) / (lds + 1024);
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    long long nb = (long long)cu_count() * per_cu;
+    if (nb > nblk) nb = nblk;
+    dim3 grid((unsigned)nb);
     if (inv)
         hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, tw, sc);
     else

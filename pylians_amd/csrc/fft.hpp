@@ -10,6 +10,9 @@ namespace pyl {
 
 bool fft_size_supported(int N);
 
+// compute units of the current device (cached)
+int cu_count();
+
 // twiddle table exp(-2 pi i t/N), t in [0, N), cached per (N, precision); device pointer
 template <typename R> const cplx<R>* twiddles(int N);
 

@@ -168,6 +168,87 @@ __device__ __forceinline__ void lds_fft(cplx<R>* s, const cplx<R>* tw, int tid) 
     fft_stages<R, N, NL, ES, T, TWSTEP, 0, 1>(s, tw, tid);
 }
 
+// ------------------------------------------------------------------ register-fed first / last stage
+// Register footprint of the first stage: B0 butterflies of RAD0 points per thread.
+template <int N, int NL, int T> struct First {
+    static constexpr Radices P = make_radices(N);
+    static constexpr int RAD = P.r[0];
+    static constexpr int STRIDE = N / RAD;
+    static constexpr int NBF = STRIDE * NL;
+    static constexpr int B = (NBF + T - 1) / T;
+    static constexpr int NST = P.n;
+};
+
+// Load the first-stage inputs of the lines held by this thread: butterfly g = tid + b*T covers
+// line l = g % NL and elements j + m*STRIDE, j = g / NL.  ld(j, l) returns one element.
+template <typename R, int N, int NL, int T, class V, class LD>
+__device__ __forceinline__ void first_load(V& v, LD&& ld, int tid) {
+    using F = First<N, NL, T>;
+#pragma unroll
+    for (int b = 0; b < F::B; ++b) {
+        const int g = tid + b * T;
+        if (F::NBF % T == 0 || g < F::NBF) {
+            const int l = g % NL, j = g / NL;
+#pragma unroll
+            for (int m = 0; m < F::RAD; ++m) v[b][m] = ld(j + m * F::STRIDE, l);
+        }
+    }
+}
+
+// First stage (no twiddles) from registers into LDS.  The caller synchronises before (LDS reuse)
+// and this function synchronises after.
+template <typename R, int N, int NL, int ES, int T, class V>
+__device__ __forceinline__ void first_stage_regs(cplx<R>* s, V& v, int tid) {
+    using F = First<N, NL, T>;
+#pragma unroll
+    for (int b = 0; b < F::B; ++b) {
+        const int g = tid + b * T;
+        if (F::NBF % T == 0 || g < F::NBF) {
+            const int l = g % NL, j = g / NL;
+            Dft<R, F::RAD>::run(v[b]);
+#pragma unroll
+            for (int m = 0; m < F::RAD; ++m) s[(j * F::RAD + m) * ES + l] = v[b][m];
+        }
+    }
+    __syncthreads();
+}
+
+// Last stage from LDS straight to the caller's store st(k, l, value) (natural order k).
+template <typename R, int N, int NL, int ES, int T, int TWSTEP, class ST>
+__device__ __forceinline__ void last_stage_store(const cplx<R>* s, const cplx<R>* tw, ST&& st, int tid) {
+    constexpr Radices P = make_radices(N);
+    constexpr int RAD = P.r[P.n - 1];
+    constexpr int NS = N / RAD;
+    constexpr int NBF = NS * NL;
+    constexpr int B = (NBF + T - 1) / T;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int g = tid + b * T;
+        if (NBF % T == 0 || g < NBF) {
+            const int l = g % NL, j = g / NL;  // j < NS: base index k = j
+            cplx<R> v[RAD];
+#pragma unroll
+            for (int m = 0; m < RAD; ++m) v[m] = s[(j + m * NS) * ES + l];
+#pragma unroll
+            for (int m = 1; m < RAD; ++m) v[m] = cmul(v[m], tw[TWSTEP * m * j * (N / (NS * RAD))]);
+            Dft<R, RAD>::run(v);
+#pragma unroll
+            for (int m = 0; m < RAD; ++m) st(j + m * NS, l, v[m]);
+        }
+    }
+}
+
+// Stages 1 .. n-2 in LDS (stage 0 done by first_stage_regs, stage n-1 by last_stage_store).
+template <typename R, int N, int NL, int ES, int T, int TWSTEP, int S, int NS>
+__device__ __forceinline__ void mid_stages(cplx<R>* s, const cplx<R>* tw, int tid) {
+    constexpr Radices P = make_radices(N);
+    if constexpr (S < P.n - 1) {
+        constexpr int RAD = P.r[S];
+        fft_stage<R, N, NL, ES, T, RAD, NS, TWSTEP>(s, tw, tid);
+        mid_stages<R, N, NL, ES, T, TWSTEP, S + 1, NS * RAD>(s, tw, tid);
+    }
+}
+
 // launch geometry shared by the host launchers and kernels
 template <typename R> struct Geo {
     static constexpr int NL = sizeof(R) == 4 ? 16 : 8;  // 128-byte column segments
