@@ -125,15 +125,19 @@ def cpu_reference_pipeline(pos, dims, BoxSize, axis, threads, MAS="CIC"):
     return res, t
 
 
-def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_floor=False):
+def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_floor=False, with_noise=False):
     """FP64 oracle of the end-to-end path at full size (C deposit, float64 r2c, C float64 mode loop).
     With with_floor, also returns the float32 floor: the exact overdensity rounded once to float32,
-    transformed with a float32 FFT (pocketfft single precision) and binned in float64."""
+    transformed with a float32 FFT (pocketfft single precision) and binned in float64.  With
+    with_noise (small grids), returns ``f32_noise_pk`` of the exact overdensity instead."""
     import scipy.fft
 
     rho = deposit_f64(pos, dims, BoxSize, MAS)
     rho /= len(pos) * 1.0 / dims ** 3
     rho -= 1.0
+    if with_noise:
+        return (_pk.Pk(None, BoxSize, sums=pk_sums_c(np.fft.rfftn(rho), dims, axis, MAS), dims=dims),
+                f32_noise_pk(rho, BoxSize, axis, MAS))
     d32 = rho.astype(np.float32) if with_floor else None
     dk = scipy.fft.rfftn(rho, workers=workers)
     del rho
@@ -144,3 +148,72 @@ def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_fl
     dk = scipy.fft.rfftn(d32, workers=workers).astype(np.complex128)
     del d32
     return o, _pk.Pk(None, BoxSize, sums=pk_sums_c(dk, dims, axis, MAS), dims=dims)
+
+
+class _Std:
+    pass
+
+
+def f32_noise_pk(delta, BoxSize, axis, MAS="CIC", samples=16, seed=0):
+    """Per-bin standard deviation of the Pk outputs caused by float32 representation noise.
+
+    A float32 pipeline (the exact overdensity rounded once to float32, a float32 r2c) adds to every
+    mode delta_k an error e_k that is white: the mean |e_k|^2 = sigma^2 is measured here with
+    pocketfft in single precision on this field (standing in for the reference's FFTW float plan,
+    Pk_library.pyx:120-133).  Real white noise of variance sigma^2 / N^3 per cell has exactly that
+    spectrum, with the Hermitian symmetry of a real field, so adding `samples` draws of it to the
+    exact spectrum and binning in float64 gives the spread the float32 representation causes in every
+    output bin.  Returned attributes Pk (kmax, 3), Pk1D, Pk2D, Pkphase are RMS deviations from the
+    noise-free bins; sigma_rel = sigma / rms |delta_k|.
+    """
+    import scipy.fft
+
+    d = np.asarray(delta, dtype=np.float64)
+    N = d.shape[0]
+    ex = np.fft.rfftn(d)
+    fl = scipy.fft.rfftn(d.astype(np.float32)).astype(np.complex128)
+    sigma2 = float(np.mean(np.abs(fl - ex) ** 2))
+    del fl
+    base = _pk.Pk(None, BoxSize, sums=pk_sums_c(ex, N, axis, MAS), dims=N)
+    rng = np.random.default_rng(seed)
+    names = ("Pk", "Pk1D", "Pk2D", "Pkphase")
+    acc = {k: np.zeros_like(getattr(base, k)) for k in names}
+    s = np.sqrt(sigma2 / N ** 3)
+    for _ in range(samples):
+        dk = ex + np.fft.rfftn(rng.normal(0.0, s, d.shape))
+        X = _pk.Pk(None, BoxSize, sums=pk_sums_c(dk, N, axis, MAS), dims=N)
+        for k in names:
+            acc[k] += (getattr(X, k) - getattr(base, k)) ** 2
+    out = _Std()
+    for k in names:
+        setattr(out, k, np.sqrt(acc[k] / samples))
+    out.sigma_rel = float(np.sqrt(sigma2 / np.mean(np.abs(ex) ** 2)))
+    return out
+
+
+def f32_noise_xpk(deltas, BoxSize, axis, MAS, samples=16, seed=0):
+    """``f32_noise_pk`` for ``XPk``: each field gets real white noise of the variance its float32
+    r2c adds (pocketfft single precision), and the float64 XPk restatement is re-run on the noisy
+    fields.  Returns RMS deviations Pk (kmax, 3, F), XPk (kmax, 3, X), Pk1D (kmax_par, F), Pk2D (n2, F)
+    and PkX2D (n2, X)."""
+    import scipy.fft
+
+    ds = [np.asarray(d, dtype=np.float64) for d in deltas]
+    N = ds[0].shape[0]
+    sd = []
+    for d in ds:
+        ex = np.fft.rfftn(d)
+        fl = scipy.fft.rfftn(d.astype(np.float32)).astype(np.complex128)
+        sd.append(np.sqrt(float(np.mean(np.abs(fl - ex) ** 2)) / N ** 3))
+    base = _pk.XPk(ds, BoxSize, axis=axis, MAS=MAS)
+    rng = np.random.default_rng(seed)
+    names = ("Pk", "XPk", "Pk1D", "Pk2D", "PkX2D")
+    acc = {k: np.zeros_like(getattr(base, k)) for k in names}
+    for _ in range(samples):
+        X = _pk.XPk([d + rng.normal(0.0, s, d.shape) for d, s in zip(ds, sd)], BoxSize, axis=axis, MAS=MAS)
+        for k in names:
+            acc[k] += (getattr(X, k) - getattr(base, k)) ** 2
+    out = _Std()
+    for k in names:
+        setattr(out, k, np.sqrt(acc[k] / samples))
+    return out

@@ -220,10 +220,20 @@ std::map<TuneKey, int> g_tune;
 bool g_tune_on = true;
 }  // namespace
 
+// Writes the overdensity into `grid` except on the float64 atomic path, which leaves rho there
+// (pass Z then applies delta = rho * a - 1 in float64).  The float32 atomic path accumulates in a
+// float64 grid and rounds delta once, like the binned path, so both give the same float32 field.
 static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
                          bool binned, void* grid, const double* dmass, hipStream_t st) {
     if (binned) return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st);
-    return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | (f64 ? DEP_GRID_F64 : 0), grid, st);
+    if (f64) return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, grid, st);
+    const int64_t cells = (int64_t)dims * dims * dims;
+    int rc = OK;
+    double* rho = (double*)workspace(WS_AUX, sizeof(double) * (size_t)cells, &rc);
+    if (rc) return rc;
+    rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, rho, st);
+    if (rc) return rc;
+    return delta_round(rho, (float*)grid, cells, dmass, (double)n, st);
 }
 
 static int tune_choice(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
@@ -347,7 +357,7 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
     if (f64)
         return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, !binned,
                                      dmass, (double)n, st);
-    return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, !binned,
+    return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, false,
                                 dmass, (double)n, st);
 }
 
@@ -371,11 +381,10 @@ extern "C" int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxS
     const bool binned = binned_supported(3, dims);
     {
         StageTimer t("adj_forward_deposit", st);
-        rc = binned ? deposit_binned(pos, nullptr, n, dims, BoxSize, scheme, false, 2, 1.0, grid, nullptr, (double)n, st)
-                    : deposit(pos, nullptr, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC, grid, st);
+        rc = deposit_delta(pos, nullptr, n, dims, BoxSize, scheme, false, binned, grid, nullptr, st);
         if (rc) return rc;
     }
-    rc = pk_from_delta<float>(grid, dims, BoxSize, 0, scheme, 0, out, keep, !binned, nullptr, (double)n, st);
+    rc = pk_from_delta<float>(grid, dims, BoxSize, 0, scheme, 0, out, keep, false, nullptr, (double)n, st);
     if (rc) return rc;
     const PkGeometry* g = pk_geometry(dims, 0, &rc);
     if (rc) return rc;

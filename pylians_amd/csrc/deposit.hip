@@ -112,6 +112,23 @@ int grid_divide(void* grid, int64_t n, bool f64, double s, hipStream_t st) {
     return OK;
 }
 
+// delta = rho * a - 1 with a = ncell / mass, evaluated in float64 and rounded once to float32
+// (Pk_snapshot.py:84-88: delta /= mean(delta); delta -= 1).
+__global__ void k_delta_round(const double* __restrict__ rho, float* __restrict__ out, long long n,
+                              const double* __restrict__ dmass, double mass, double ncell) {
+    const double a = ncell / (dmass ? dmass[0] : mass);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (float)(rho[i] * a - 1.0);
+}
+
+int delta_round(const double* rho, float* out, int64_t n, const double* dmass, double mass, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_delta_round, dim3(blocks), dim3(256), 0, st, rho, out, (long long)n, dmass, mass, (double)n);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
 template <int S, typename G, bool HASW, int AXES>
 static int launch_atomic(const float* pos, const float* W, int64_t n, int dims, double inv, double mult, void* grid,
                          hipStream_t st) {

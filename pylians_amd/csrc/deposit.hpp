@@ -21,6 +21,8 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
 
 // grid /= s (float32 or float64 grid of n cells), as numpy's `number2 /= 2.0` (MAS_library.pyx:89)
 int grid_divide(void* grid, int64_t n, bool f64, double s, hipStream_t st);
+// out[i] = (float)(rho[i] * n / mass - 1) in float64 (mass = dmass[0] when dmass is non-null).
+int delta_round(const double* rho, float* out, int64_t n, const double* dmass, double mass, hipStream_t st);
 
 // Cell-binned LDS deposit (3-D grids with dims % 16 == 0).  mode: 0 = grid += rho, 1 = grid = rho,
 // 2 = grid = rho * N^3 / mass - 1 with mass = dmass[0] when dmass != nullptr, else `mass`.

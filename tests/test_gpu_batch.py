@@ -37,8 +37,8 @@ def test_graph_batch_matches_eager_and_oracle(gpu):
             assert np.allclose(g.Pk, e.Pk, rtol=1e-12, atol=0)
             assert np.allclose(g.Pk2D, e.Pk2D, rtol=1e-12, atol=1e-300)
     for g, j in zip(res, jobs):
-        o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_floor=True)
-        assert_pk_close(g, o, rtol=1e-5, floor=fl)
+        o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_noise=True)
+        assert_pk_close(g, o, rtol=1e-5, noise=fl)
     b.close()
 
 
@@ -48,8 +48,8 @@ def test_batch_without_graph(gpu):
     b = batch.PkBatch(jobs, graph=False)
     b.run()
     for g, j in zip(b.results(), jobs):
-        o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_floor=True)
-        assert_pk_close(g, o, rtol=1e-5, floor=fl)
+        o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_noise=True)
+        assert_pk_close(g, o, rtol=1e-5, noise=fl)
 
 
 def test_autotuner_cache_roundtrip(gpu, tmp_path):

@@ -19,8 +19,8 @@ def test_uniform_128_cic_axis0(pipe):
     N, L = 128, 1000.0
     pos = synth.uniform(N ** 3, L, seed=1)
     g = pipe.PkParticles(pos, L, N, MAS="CIC", axis=0)
-    o, floor = oracle_pk_particles(pos, L, N, "CIC", 0, with_floor=True)
-    errs = assert_pk_close(g, o, rtol=1e-5, floor=floor)
+    o, noise = oracle_pk_particles(pos, L, N, "CIC", 0, with_noise=True)
+    errs = assert_pk_close(g, o, rtol=1e-5, noise=noise)
     print("uniform 128^3 CIC: max rel err vs FP64 oracle", errs)
 
 
@@ -29,8 +29,8 @@ def test_zeldovich_rsd(pipe, MAS):
     n, N, L = 64, 64, 250.0
     pos = synth.zeldovich(n, L, seed=2, rms_disp=1.25, sigma_v=75.0, axis=0)
     g = pipe.PkParticles(pos, L, N, MAS=MAS, axis=0)
-    o, floor = oracle_pk_particles(pos, L, N, MAS, 0, with_floor=True)
-    errs = assert_pk_close(g, o, rtol=1e-5, floor=floor)
+    o, noise = oracle_pk_particles(pos, L, N, MAS, 0, with_noise=True)
+    errs = assert_pk_close(g, o, rtol=1e-5, noise=noise)
     print("zeldovich 64^3 %s: max rel err vs FP64 oracle" % MAS, errs)
 
 
@@ -42,5 +42,5 @@ def test_float64_path_and_weights(pipe):
     o = oracle_pk_particles(pos, L, N, "CIC", 2, W=W)
     assert_pk_close(g, o, rtol=1e-10)
     g32 = pipe.PkParticles(pos, L, N, MAS="CIC", axis=2, W=W)
-    _, floor = oracle_pk_particles(pos, L, N, "CIC", 2, W=W, with_floor=True)
-    assert_pk_close(g32, o, rtol=1e-5, floor=floor)
+    _, noise = oracle_pk_particles(pos, L, N, "CIC", 2, W=W, with_noise=True)
+    assert_pk_close(g32, o, rtol=1e-5, noise=noise)

@@ -1,0 +1,69 @@
+"""The float32-noise parity criterion used by the GPU tests: a float32 pipeline (pocketfft single
+precision) passes it on fields where 1e-5 is out of reach for float32, and planted mistakes fail it."""
+import copy
+
+import numpy as np
+import pytest
+import scipy.fft
+
+from oracle import cpu as C
+from oracle import pk as P
+from pylians_amd import synth
+from tests._gpuref import assert_pk_close, oracle_pk, oracle_pk_particles
+
+
+def _f32_pipeline(pos, L, N, MAS, axis):
+    rho = C.deposit_f64(pos, N, L, MAS)
+    d = (rho / (len(pos) / N ** 3) - 1.0).astype(np.float32)
+    dk = scipy.fft.rfftn(d).astype(np.complex128)
+    return P.Pk(None, L, sums=C.pk_sums_c(dk, N, axis, MAS), dims=N)
+
+
+@pytest.mark.parametrize("MAS", ["CIC", "PCS"])
+def test_float32_pipeline_passes_where_1e5_fails(MAS):
+    pos = synth.zeldovich(32, 125.0, seed=5, rms_disp=0.6, sigma_v=40.0, axis=0)
+    o, nz = oracle_pk_particles(pos, 125.0, 32, MAS, 0, with_noise=True)
+    f = _f32_pipeline(pos, 125.0, 32, MAS, 0)
+    e = np.abs(f.Pk - o.Pk) / np.abs(o.Pk[:, :1])
+    e2 = (np.abs(f.Pk2D - o.Pk2D) / np.where(o.Pk2D == 0, 1, np.abs(o.Pk2D)))[1:]
+    assert max(e.max(), e2.max()) > 1e-5       # float32 cannot reach 1e-5 in every bin here
+    assert_pk_close(f, o, rtol=1e-5, noise=nz)
+
+
+def test_planted_errors_fail():
+    pos = synth.zeldovich(32, 125.0, seed=6, rms_disp=0.6, sigma_v=40.0, axis=0)
+    o, nz = oracle_pk_particles(pos, 125.0, 32, "CIC", 0, with_noise=True)
+    f = _f32_pipeline(pos, 125.0, 32, "CIC", 0)
+    assert_pk_close(f, o, rtol=1e-5, noise=nz)
+    # a relative 3e-5 error in one well-sampled bin of P0
+    bad = copy.deepcopy(f)
+    bad.Pk[5, 0] *= 1.0 + 3e-5
+    with pytest.raises(AssertionError):
+        assert_pk_close(bad, o, rtol=1e-5, noise=nz)
+    # a missing deconvolution (MAS=None instead of CIC) shifts the high-k bins far beyond the noise
+    rho = C.deposit_f64(pos, 32, 125.0, "CIC")
+    d = rho / (len(pos) / 32 ** 3) - 1.0
+    nowin = oracle_pk(d, 125.0, 0, "None")
+    nowin.Nmodes3D = o.Nmodes3D
+    with pytest.raises(AssertionError):
+        assert_pk_close(nowin, o, rtol=1e-5, noise=nz)
+    # a quadrupole with the wrong sign of mu^2 in L2 (mistake in P2 only)
+    wrong = copy.deepcopy(f)
+    wrong.Pk[:, 1] = -wrong.Pk[:, 1]
+    with pytest.raises(AssertionError):
+        assert_pk_close(wrong, o, rtol=1e-5, noise=nz)
+
+
+def test_noise_model_is_calibrated():
+    """In the bins whose float32 error exceeds 1e-5, pocketfft's errors stay within a few sigma."""
+    worst = 0.0
+    for seed in range(3):
+        pos = synth.zeldovich(32, 125.0, seed=20 + seed, rms_disp=0.6, sigma_v=40.0, axis=0)
+        o, nz = oracle_pk_particles(pos, 125.0, 32, "PCS", 0, with_noise=True)
+        f = _f32_pipeline(pos, 125.0, 32, "PCS", 0)
+        P0 = np.abs(o.Pk[:, 0])[:, None]
+        e = np.abs(f.Pk - o.Pk) / P0
+        m = e > 1e-5
+        if m.any():
+            worst = max(worst, float(np.max(e[m] / (nz.Pk / P0)[m])))
+    assert worst < 4.0, worst
