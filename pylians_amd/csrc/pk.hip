@@ -12,6 +12,7 @@
 // Per-mode arithmetic is in the working precision R; every shell sum is float64.
 #include <math.h>
 
+#include <algorithm>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -202,10 +203,11 @@ constexpr int hist_bytes(int N, int NL) {
     return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * 4 * 8 + (N / 2 + 1) * 8 +
            NL * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 8;
 }
-// lines per block for the binning kernel: as many as fit beside the histograms
+// lines per block for the binning kernel: at most 8, and few enough that two blocks share a CU
+// (160 KB of LDS) beside their histograms
 template <typename R, int N> constexpr int nl_bin() {
-    int nl = Geo<R>::NL;
-    while (nl > 2 && N * nl * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + 8) + hist_bytes(N, nl) > 150 * 1024)
+    int nl = 8;
+    while (nl > 2 && N * nl * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + 8) + hist_bytes(N, nl) > 80 * 1024)
         nl /= 2;
     return nl;
 }
@@ -214,8 +216,10 @@ template <int N, int NL> constexpr int threads_bin() { return clamp_i(N * NL / 1
 // The last FFT pass of the forward transform, fused with the window deconvolution and the
 // binning.  The pass runs along x (last = 0; column (u = y, kz)) or y (last = 1; column
 // (u = x, kz)), chosen so that k_par is constant along each column unless the line of sight is z.
-// Persistent blocks own either one u (all kz blocks: "fixk = 0", k_par = |k_u|) or one kz block
-// and a range of u ("fixk = 1", k_par = kz), so their 2-D histogram window stays in LDS.
+// Blocks own either one u and a range of kz blocks ("fixk = 0", k_par = |k_u|) or one kz block
+// and a range of u ("fixk = 1", k_par = kz), so their 2-D histogram window stays in LDS.  A block
+// walks its column blocks in order; the first-stage inputs of the next one are loaded into
+// registers while the current one is transformed and binned.
 template <typename R, int N, int NL, int T>
 __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
                                                  long long P, int last, int axis, int fixk, int nchunk, int flags,
@@ -226,6 +230,7 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     constexpr int TPC = T / NL;             // threads per column
     constexpr int E = (N + TPC - 1) / TPC;  // consecutive elements per thread in the epilogue
     static_assert(T % NL == 0, "bad binning geometry");
+    using F = First<N, NL, T>;
     __shared__ cplx<R> s[N * NL];
     __shared__ cplx<R> tw[N];
     __shared__ double win[N];
@@ -250,31 +255,51 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     const int l = tid % NL;
     const int jrow = tid / NL;
     const bool do1 = flags & PK_1D, do2 = flags & PK_2D, doph = flags & PK_PHASE;
-    // work items of this block
-    int u0, u1, kzb0, kzb1;
+    // column blocks of this block: items it = 0 .. nit-1
+    int u0, kzb0, nit;
     if (fixk) {
         const int kzb = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
         const int U = (N + nchunk - 1) / nchunk;
         u0 = ch * U;
-        u1 = min(N, u0 + U);
+        nit = max(0, min(N, u0 + U) - u0);
         kzb0 = kzb;
-        kzb1 = kzb + 1;
     } else {
-        u0 = blockIdx.x;
-        u1 = u0 + 1;
-        kzb0 = 0;
-        kzb1 = nkzb;
+        const int KB = (nkzb + nchunk - 1) / nchunk;
+        u0 = blockIdx.x / nchunk;
+        kzb0 = (blockIdx.x % nchunk) * KB;
+        nit = max(0, min(nkzb, kzb0 + KB) - kzb0);
     }
+    auto item = [&](int it, int& u, int& kzz) {
+        u = fixk ? u0 + it : u0;
+        kzz = (fixk ? kzb0 : kzb0 + it) * NL + l;
+    };
+    auto fetch = [&](int it, cplx<R> (&v)[F::B][F::RAD]) {
+        int u, kzz;
+        item(it, u, kzz);
+        const bool ok = kzz < P;
+        const long long b = (long long)u * UST + kzz;
+        first_load<R, N, NL, T>(v, [&](int j, int) { return ok ? src[b + j * EST] : mk<R>(0, 0); }, tid);
+    };
+    cplx<R> nxt[F::B][F::RAD];
+    if (nit > 0) fetch(0, nxt);
     const int kpar_u = fixk ? 0 : abs(kfreq(u0, N));
-    for (int u = u0; u < u1; ++u) {
-        for (int kzb = kzb0; kzb < kzb1; ++kzb) {
-            const int kzz = kzb * NL + l;
+    for (int it = 0; it < nit; ++it) {
+        {
+            cplx<R> cur[F::B][F::RAD];
+#pragma unroll
+            for (int b = 0; b < F::B; ++b)
+#pragma unroll
+                for (int m = 0; m < F::RAD; ++m) cur[b][m] = nxt[b][m];
+            if (it + 1 < nit) fetch(it + 1, nxt);
+            __syncthreads();   // the previous epilogue has read s
+            first_stage_regs<R, N, NL, NL, T>(s, cur, tid);
+        }
+        fft_stages<R, N, NL, NL, T, 1, 1, F::RAD>(s, tw, tid);
+        {
+            int u, kzz;
+            item(it, u, kzz);
             const bool valid = kzz < P;
             const long long base = (long long)u * UST + kzz;
-            __syncthreads();
-            for (int j = jrow; j < N; j += TPC) s[j * NL + l] = valid ? src[base + j * EST] : mk<R>(0, 0);
-            __syncthreads();
-            lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
             if (!valid) continue;
             const int kz = kzz;
             const int ku = kfreq(u, N);
@@ -386,11 +411,15 @@ static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int ax
     const int last = pk_last_axis(axis);
     const int fixk = axis == 2 ? 1 : 0;
     const int nkzb = (int)((P + NL - 1) / NL);
-    int nchunk = 1, grid = N;
+    // about two blocks per CU, all resident at once; each walks its column blocks with prefetch
+    const int want = 2 * num_cus();
+    int nchunk, grid;
     if (fixk) {
-        nchunk = (num_cus() * 4 + nkzb - 1) / nkzb;
-        if (nchunk > N) nchunk = N;
+        nchunk = std::min(N, std::max(1, (want + nkzb - 1) / nkzb));
         grid = nkzb * nchunk;
+    } else {
+        nchunk = std::min(nkzb, std::max(1, (want + N - 1) / N));
+        grid = N * nchunk;
     }
     const int nper = d.kmax_per + 1;
     size_t shm = sizeof(double) * (4 * (size_t)d.n3() + d.n1() + (size_t)(fixk ? NL : 1) * nper);
