@@ -157,7 +157,10 @@ template <typename R>
 static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int mas_p, int flags, double* out,
                          cplx<R>* keep, bool affine, const double* dmass, double mass, hipStream_t st) {
     int rc = OK;
-    const int64_t P = half_len(dims);
+    // rows of the internal spectrum padded to 128 bytes: the strided column passes then read whole
+    // cache lines (kz = N/2+1 .. P-1 are never touched)
+    constexpr int64_t ALIGN = 128 / (int64_t)sizeof(cplx<R>);
+    const int64_t P = (half_len(dims) + ALIGN - 1) / ALIGN * ALIGN;
     const PkGeometry* g = pk_geometry(dims, axis, &rc);
     if (rc) return rc;
     rc = check_modes(g);

@@ -135,14 +135,15 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
     }
 }
 
-// Column FFT over a half spectrum.  Column c (of ncols) starts at (c / CW) * CS + c % CW and
-// its N elements are EST apart.  Pass Y: CW = P, CS = N*P, EST = P.  Pass X: CW = ncols, EST = N*P.
+// Column FFT over a half spectrum with row pitch P (kz < W = N/2+1 hold data).  Column c (of
+// ncols) starts at (c / CW) * CS + c % CW and its N elements are EST apart.  Pass Y: CW = P,
+// CS = N*P, EST = P.  Pass X: CW = ncols, EST = N*P.  Columns with c % P >= W are skipped.
 // Persistent blocks walk column blocks of NL columns; while one block is transformed the first-stage
 // inputs of the next one are already in flight into registers, and the last stage stores straight
 // from registers.  With a single radix stage the transform never touches LDS.
 template <typename R, int N, int NL, int T, bool INV>
 __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, long long ncols,
-                                             long long CW, long long CS, long long EST,
+                                             long long CW, long long CS, long long EST, long long PW, int W,
                                              const cplx<R>* __restrict__ twg, R sc) {
     static_assert(T % NL == 0, "T must be a multiple of NL");
     using F = First<N, NL, T>;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, lo
     const long long nblk = (ncols + NL - 1) / NL;
     auto col_base = [&](long long cb, bool& valid) {
         const long long c = cb * NL + l;
-        valid = c < ncols;
+        valid = c < ncols && (int)(c % PW) < W;   // kz = c % PW; columns kz >= W are row padding
         return valid ? (c / CW) * CS + (c % CW) : 0;
     };
     cplx<R> nxt[F::B][F::RAD];
@@ -328,9 +329,11 @@ static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bo
     if (nb > nblk) nb = nblk;
     dim3 grid((unsigned)nb);
     if (inv)
-        hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, tw, sc);
+        hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST,
+                           (long long)P, N / 2 + 1, tw, sc);
     else
-        hipLaunchKernelGGL((k_cols<R, N, NL, T, false>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, tw, sc);
+        hipLaunchKernelGGL((k_cols<R, N, NL, T, false>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST,
+                           (long long)P, N / 2 + 1, tw, sc);
     PYL_HIP(hipGetLastError());
     return OK;
 }

@@ -21,6 +21,7 @@
 #include "fft.hpp"
 #include "fft_engine.hpp"
 #include "pk.hpp"
+#include "util.hpp"
 
 namespace pyl {
 
@@ -205,35 +206,102 @@ constexpr int hist_bytes(int N, int NL) {
 }
 // lines per block for the binning kernel: at most 8, and few enough that two blocks share a CU
 // (160 KB of LDS) beside their histograms
+constexpr int threads_bin_c(int N, int NL) { return clamp_i(N * NL / 16, 64, 512); }
 template <typename R, int N> constexpr int nl_bin() {
     int nl = 8;
-    while (nl > 2 && N * nl * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + 8) + hist_bytes(N, nl) > 80 * 1024)
+    while (nl > 2 && (N + threads_bin_c(N, nl) / nl) * nl * (int)sizeof(cplx<R>) +
+                             N * (int)(sizeof(cplx<R>) + sizeof(R)) + hist_bytes(N, nl) > 80 * 1024)
         nl /= 2;
     return nl;
 }
-template <int N, int NL> constexpr int threads_bin() { return clamp_i(N * NL / 16, 64, 512); }
+template <int N, int NL> constexpr int threads_bin() { return threads_bin_c(N, NL); }
+
+// floor(sqrt(v)) for 0 <= v < 2^24 (float estimate within one of the root, then corrected)
+__device__ __forceinline__ int isqrt24(int v) {
+    int r = (int)__builtin_amdgcn_sqrtf((float)v);
+    r -= (r * r > v) ? 1 : 0;
+    r += ((r + 1) * (r + 1) <= v) ? 1 : 0;
+    return r;
+}
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
+// LDS row of FFT output k in the binning kernel.  The epilogue pairs rows k and N-k (same |k|);
+// thread jrow of a column owns the pairs u in [jrow*EU, (jrow+1)*EU).  With EU odd, the rows of the
+// 8 threads of a wave that share a step are an odd number of 64-byte rows apart and spread over all
+// LDS banks; with EU even, one spare row per group restores that.
+template <int N, int EU> struct RowMap {
+    static constexpr int PADR = (EU % 2 == 0) ? 1 : 0;
+    static constexpr int GL = (N / 2) / EU + 1;   // groups of the lower half k <= N/2
+    static constexpr int GU = (N / 2) / EU;
+    static constexpr int ROWS = N + PADR * (GL + GU + 1);
+    static __device__ __forceinline__ int row(int k) {
+        if (PADR == 0) return k;
+        if (k <= N / 2) return k + k / EU;
+        return k + GL + GU - (N - k) / EU;
+    }
+};
+
+// Last radix stage of the forward transform, written back to LDS in the RowMap order.
+template <typename R, int N, int NL, int T, class RM>
+__device__ __forceinline__ void last_stage_mapped(cplx<R>* s, const cplx<R>* tw, int tid) {
+    constexpr Radices PL = make_radices(N);
+    constexpr int RAD = PL.r[PL.n - 1];
+    constexpr int NS = N / RAD;
+    constexpr int NBF = NS * NL;
+    constexpr int B = (NBF + T - 1) / T;
+    cplx<R> v[B][RAD];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int g = tid + b * T;
+        if (NBF % T == 0 || g < NBF) {
+            const int l = g % NL, j = g / NL;
+#pragma unroll
+            for (int m = 0; m < RAD; ++m) v[b][m] = s[(j + m * NS) * NL + l];
+            if (NS > 1) {
+#pragma unroll
+                for (int m = 1; m < RAD; ++m) v[b][m] = cmul(v[b][m], tw[m * j]);
+            }
+            Dft<R, RAD>::run(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int g = tid + b * T;
+        if (NBF % T == 0 || g < NBF) {
+            const int l = g % NL, j = g / NL;
+#pragma unroll
+            for (int m = 0; m < RAD; ++m) s[RM::row(j + m * NS) * NL + l] = v[b][m];
+        }
+    }
+    __syncthreads();
+}
 
 // The last FFT pass of the forward transform, fused with the window deconvolution and the
 // binning.  The pass runs along x (last = 0; column (u = y, kz)) or y (last = 1; column
-// (u = x, kz)), chosen so that k_par is constant along each column unless the line of sight is z.
-// Blocks own either one u and a range of kz blocks ("fixk = 0", k_par = |k_u|) or one kz block
-// and a range of u ("fixk = 1", k_par = kz), so their 2-D histogram window stays in LDS.  A block
-// walks its column blocks in order; the first-stage inputs of the next one are loaded into
-// registers while the current one is transformed and binned.
+// (u = x, kz)), chosen so that k_par is constant along each column (line of sight x or y: k_par =
+// k_u; z: k_par = kz).  Blocks own either one u and a range of kz blocks ("fixk = 0") or one kz
+// block and a range of u ("fixk = 1"), so their 2-D histogram window stays in LDS.  A block walks
+// its column blocks in order; the first-stage inputs of the next one are loaded into registers
+// while the current one is transformed and binned.  The spectrum has row pitch P (kz < W valid);
+// the optional window-corrected copy `keep` has pitch W.
 template <typename R, int N, int NL, int T>
 __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
-                                                 long long P, int last, int axis, int fixk, int nchunk, int flags,
-                                                 int kmax_par, int kmax_per, int n1, int n3,
+                                                 long long P, int W, int last, int axis, int fixk, int nchunk,
+                                                 int flags, int kmax_par, int kmax_per, int n1, int n3,
                                                  const double* __restrict__ wing, const cplx<R>* __restrict__ twg,
-                                                 double* __restrict__ g3, double* __restrict__ g1,
-                                                 double* __restrict__ g2) {
-    constexpr int TPC = T / NL;             // threads per column
-    constexpr int E = (N + TPC - 1) / TPC;  // consecutive elements per thread in the epilogue
+                                                 double* __restrict__ part, int RS) {
+    constexpr int TPC = T / NL;              // threads per column
+    constexpr int U = N / 2 + 1;             // row pairs (k, N-k), k = 0 .. N/2 (k = 0, N/2 alone)
+    constexpr int EU = (U + TPC - 1) / TPC;  // consecutive pairs per thread in the epilogue
     static_assert(T % NL == 0, "bad binning geometry");
     using F = First<N, NL, T>;
-    __shared__ cplx<R> s[N * NL];
+    using RM = RowMap<N, EU>;
+    constexpr bool ONE = F::NST == 1;
+    __shared__ cplx<R> s[(RM::ROWS > N ? RM::ROWS : N) * NL];
     __shared__ cplx<R> tw[N];
-    __shared__ double win[N];
+    __shared__ R win[N];
     extern __shared__ double hs[];
     const int nper = kmax_per + 1;
     double* h3 = hs;             // n3 * 4
@@ -243,15 +311,17 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     const int tid = threadIdx.x;
     for (int i = tid; i < N; i += T) {
         tw[i] = twg[i];
-        win[i] = wing[i];
+        win[i] = (R)wing[i];
     }
     for (int i = tid; i < 4 * n3 + n1 + n2w; i += T) hs[i] = 0.0;
 
     const int middle = N / 2;
     const bool even = (N % 2) == 0;
-    const int nkzb = (int)((P + NL - 1) / NL);
+    const int nkzb = (W + NL - 1) / NL;
     const long long EST = last == 0 ? (long long)N * P : P;   // element stride along the pass axis
     const long long UST = last == 0 ? P : (long long)N * P;   // stride of the outer coordinate u
+    const long long KEST = last == 0 ? (long long)N * W : W;  // the same in `keep`
+    const long long KUST = last == 0 ? W : (long long)N * W;
     const int l = tid % NL;
     const int jrow = tid / NL;
     const bool do1 = flags & PK_1D, do2 = flags & PK_2D, doph = flags & PK_PHASE;
@@ -259,9 +329,9 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     int u0, kzb0, nit;
     if (fixk) {
         const int kzb = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
-        const int U = (N + nchunk - 1) / nchunk;
-        u0 = ch * U;
-        nit = max(0, min(N, u0 + U) - u0);
+        const int UC = (N + nchunk - 1) / nchunk;
+        u0 = ch * UC;
+        nit = max(0, min(N, u0 + UC) - u0);
         kzb0 = kzb;
     } else {
         const int KB = (nkzb + nchunk - 1) / nchunk;
@@ -276,13 +346,12 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     auto fetch = [&](int it, cplx<R> (&v)[F::B][F::RAD]) {
         int u, kzz;
         item(it, u, kzz);
-        const bool ok = kzz < P;
+        const bool ok = kzz < W;
         const long long b = (long long)u * UST + kzz;
         first_load<R, N, NL, T>(v, [&](int j, int) { return ok ? src[b + j * EST] : mk<R>(0, 0); }, tid);
     };
     cplx<R> nxt[F::B][F::RAD];
     if (nit > 0) fetch(0, nxt);
-    const int kpar_u = fixk ? 0 : abs(kfreq(u0, N));
     for (int it = 0; it < nit; ++it) {
         {
             cplx<R> cur[F::B][F::RAD];
@@ -292,98 +361,172 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
                 for (int m = 0; m < F::RAD; ++m) cur[b][m] = nxt[b][m];
             if (it + 1 < nit) fetch(it + 1, nxt);
             __syncthreads();   // the previous epilogue has read s
-            first_stage_regs<R, N, NL, NL, T>(s, cur, tid);
-        }
-        fft_stages<R, N, NL, NL, T, 1, 1, F::RAD>(s, tw, tid);
-        {
-            int u, kzz;
-            item(it, u, kzz);
-            const bool valid = kzz < P;
-            const long long base = (long long)u * UST + kzz;
-            if (!valid) continue;
-            const int kz = kzz;
-            const int ku = kfreq(u, N);
-            const double cuz = win[u] * win[kzz];
-            const bool zplane = (kz == 0) || (kz == middle && even);
-            int cur3 = -1, cur2 = -1;
-            double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
-            const int j0 = jrow * E;
-            for (int q = 0; q < E; ++q) {
-                const int j = j0 + q;
-                if (N % TPC != 0 && j >= N) break;
-                const int kj = kfreq(j, N);
-                const int kx = last == 0 ? kj : ku;
-                const int ky = last == 0 ? ku : kj;
-                cplx<R> v = s[j * NL + l];
-                const bool ind = !(zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0)));
-                if (!ind) {
-                    if (keep) keep[base + j * EST] = v;
-                    continue;
-                }
-                const R fac = (R)(win[j] * cuz);
-                v = scale(v, fac);
-                if (keep) keep[base + j * EST] = v;
-                const int k2 = kx * kx + ky * ky + kz * kz;
-                const int bin = isqrt_i(k2);
-                const int kps = axis == 0 ? kx : (axis == 1 ? ky : kz);
-                const R mu2 = k2 == 0 ? (R)0 : (R)(kps * kps) / (R)k2;
-                const R d2 = v.x * v.x + v.y * v.y;
-                const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
-                const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
-                if (bin != cur3) {
-                    if (cur3 >= 0) {
-                        atomicAdd(&h3[4 * cur3 + 0], a0);
-                        atomicAdd(&h3[4 * cur3 + 1], a2);
-                        atomicAdd(&h3[4 * cur3 + 2], a4);
-                        if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+            if constexpr (ONE) {
+                // a single radix stage: straight from registers to the padded rows
+#pragma unroll
+                for (int b = 0; b < F::B; ++b) {
+                    const int g = tid + b * T;
+                    if (F::NBF % T == 0 || g < F::NBF) {
+                        const int jj = g / NL;
+                        Dft<R, F::RAD>::run(cur[b]);
+#pragma unroll
+                        for (int m = 0; m < F::RAD; ++m) s[RM::row(jj * F::RAD + m) * NL + l] = cur[b][m];
                     }
-                    cur3 = bin;
-                    a0 = a2 = a4 = ap = 0.0;
                 }
-                a0 += (double)d2;
-                a2 += (double)(d2 * L2);
-                a4 += (double)(d2 * L4);
-                if (doph) {
-                    const R ph = atan2(v.x, sqrt(d2));
-                    ap += (double)(ph * ph);
-                }
-                if (do1 && k2 <= middle * middle) b1 += (double)d2;   // k_par is constant on the column
-                if (do2) {
-                    const int kper = isqrt_i(axis == 0 ? ky * ky + kz * kz
-                                                        : (axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
-                    const int w2 = (fixk ? l * nper : 0) + kper;
-                    if (w2 != cur2) {
-                        if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
-                        cur2 = w2;
-                        b2 = 0.0;
-                    }
-                    b2 += (double)d2;
-                }
+                __syncthreads();
+            } else {
+                first_stage_regs<R, N, NL, NL, T>(s, cur, tid);
             }
-            if (cur3 >= 0) {
-                atomicAdd(&h3[4 * cur3 + 0], a0);
-                atomicAdd(&h3[4 * cur3 + 1], a2);
-                atomicAdd(&h3[4 * cur3 + 2], a4);
-                if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
-            }
-            if (do1 && b1 != 0.0) {
-                const int kpc = fixk ? kz : kpar_u;   // constant k_par of this column
-                atomicAdd(&h1[kpc], b1);
-            }
-            if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
         }
+        if constexpr (!ONE) {
+            mid_stages<R, N, NL, NL, T, 1, 1, F::RAD>(s, tw, tid);
+            last_stage_mapped<R, N, NL, T, RM>(s, tw, tid);
+        }
+        int u, kzz;
+        item(it, u, kzz);
+        if (kzz >= W) continue;
+        const int kz = kzz;
+        const int ku = kfreq(u, N);
+        const R cuz = win[u] * win[kzz];
+        const bool zplane = (kz == 0) || (kz == middle && even);
+        // k_par is constant on the column: k_u (line of sight x or y) or kz
+        const int kps = axis == 2 ? kz : ku;
+        const int c0 = ku * ku + kz * kz;
+        const int cper = c0 - kps * kps;
+        const R kps2 = (R)(kps * kps);
+        const long long kbase = (long long)u * KUST + kzz;
+        int cur3 = -1, cur2 = -1;
+        double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
+        // pairs of rows with opposite k_j: the same |k|, k_perp, mu^2 and window factor
+        for (int q = 0; q < EU; ++q) {
+            const int uu = jrow * EU + q;
+            if (U % TPC != 0 && uu >= U) break;
+            const int ja = uu, jb = (N - uu) % N;
+            const bool pair = jb != ja;
+            const cplx<R> va = s[RM::row(ja) * NL + l];
+            const cplx<R> vb = pair ? s[RM::row(jb) * NL + l] : mk<R>(0, 0);
+            // independent modes on the kz = 0 and Nyquist planes (Pk_library.pyx:322-330)
+            bool ia = true, ib = pair;
+            if (zplane) {
+                const int xa = last == 0 ? uu : ku, ya = last == 0 ? ku : uu;
+                const int xb = last == 0 ? -uu : ku, yb = last == 0 ? ku : -uu;
+                // k_j = N/2 is +N/2 (kfreq); -uu for the partner row N - uu
+                ia = !(xa < 0 || ((xa == 0 || (xa == middle && even)) && ya < 0));
+                ib = pair && !(xb < 0 || ((xb == 0 || (xb == middle && even)) && yb < 0));
+            }
+            const R fac = win[uu] * cuz;
+            if (keep) {
+                keep[kbase + ja * KEST] = ia ? scale(va, fac) : va;
+                if (pair) keep[kbase + jb * KEST] = ib ? scale(vb, fac) : vb;
+            }
+            if (!ia && !ib) continue;
+            const R ra = va.x * va.x + va.y * va.y;
+            const R rb = vb.x * vb.x + vb.y * vb.y;
+            const R r2 = (ia ? ra : (R)0) + (ib ? rb : (R)0);
+            const int k2 = c0 + uu * uu;
+            const int bin = isqrt24(k2);
+            const R mu2 = k2 == 0 ? (R)0 : kps2 * recip((R)k2);
+            const R d2 = r2 * (fac * fac);
+            const R L2 = ((R)1.5 * mu2 - (R)0.5);
+            const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
+            if (bin != cur3) {
+                if (cur3 >= 0) {
+                    atomicAdd(&h3[4 * cur3 + 0], a0);
+                    atomicAdd(&h3[4 * cur3 + 1], a2);
+                    atomicAdd(&h3[4 * cur3 + 2], a4);
+                    if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+                }
+                cur3 = bin;
+                a0 = a2 = a4 = ap = 0.0;
+            }
+            const double dd = (double)d2;
+            a0 += dd;
+            a2 += (double)(d2 * L2);
+            a4 += (double)(d2 * L4);
+            if (doph) {
+                // the phase of a mode is unchanged by the (positive) window factor
+                R pp = (R)0;
+                if (ia) {
+                    const R ph = atan2(va.x, sqrt(ra));
+                    pp += ph * ph;
+                }
+                if (ib) {
+                    const R ph = atan2(vb.x, sqrt(rb));
+                    pp += ph * ph;
+                }
+                ap += (double)pp;
+            }
+            if (do1 && k2 <= middle * middle) b1 += dd;
+            if (do2) {
+                const int w2 = (fixk ? l * nper : 0) + isqrt24(cper + uu * uu);
+                if (w2 != cur2) {
+                    if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
+                    cur2 = w2;
+                    b2 = 0.0;
+                }
+                b2 += dd;
+            }
+        }
+        if (cur3 >= 0) {
+            atomicAdd(&h3[4 * cur3 + 0], a0);
+            atomicAdd(&h3[4 * cur3 + 1], a2);
+            atomicAdd(&h3[4 * cur3 + 2], a4);
+            if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+        }
+        if (do1 && b1 != 0.0) atomicAdd(&h1[abs(kps)], b1);
+        if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
     }
+    // this block's histograms -> its row of the partials (summed by k_bin_reduce31 / k_bin_reduce2)
     __syncthreads();
-    for (int i = tid; i < 4 * n3; i += T)
-        if (hs[i] != 0.0) atomicAdd(&g3[i], hs[i]);
-    for (int i = tid; i < n1; i += T)
-        if (h1[i] != 0.0) atomicAdd(&g1[i], h1[i]);
-    for (int i = tid; i < n2w; i += T) {
-        if (h2[i] == 0.0) continue;
-        const int kper = i % nper;
-        const int kpar = fixk ? kzb0 * NL + i / nper : kpar_u;
-        if (kpar <= kmax_par) atomicAdd(&g2[(long long)(kmax_par + 1) * kper + kpar], h2[i]);
+    double* row = part + (long long)blockIdx.x * RS;
+    for (int i = tid; i < 4 * n3 + n1 + n2w; i += T) row[i] = hs[i];
+}
+
+// Sum the per-block histogram rows of k_last_bin into the pass sums.
+// 3-D and 1-D entries (k_bin_reduce31): 64 entries per workgroup, the blocks split over the 4 waves
+// and over gridDim.y slices; each slice adds its total with one atomic.
+__global__ __launch_bounds__(256) void k_bin_reduce31(const double* __restrict__ part, int RS, int nblocks, int n3,
+                                                      int n1, double* __restrict__ g3, double* __restrict__ g1) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const int n31 = 4 * n3 + n1;
+    const int step = 4 * gridDim.y;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n31) {
+        int b = blockIdx.y * 4 + w, q = 0;
+        for (; b < nblocks; b += step, q = (q + 1) & 3) a[q] += part[(long long)b * RS + i];
     }
+    red[w][lane] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (w == 0 && i < n31) {
+        const double t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        if (t != 0.0) atomicAdd(i < 4 * n3 ? &g3[i] : &g1[i - 4 * n3], t);
+    }
+}
+
+// 2-D entries (k_bin_reduce2): (k_per, k_par) comes from the blocks whose window holds k_par:
+// fixk = 0, the blocks of u = k_par and u = N - k_par (k_par = |k_u|); fixk = 1, the blocks of kz
+// block k_par / NL, row k_par % NL.
+__global__ void k_bin_reduce2(const double* __restrict__ part, int RS, int nblocks, int n31, int nper, int kmax_par,
+                              int fixk, int nchunk, int NL, int N, double* __restrict__ g2) {
+    const long long i2 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i2 >= (long long)(kmax_par + 1) * nper) return;
+    const int kpar = (int)(i2 % (kmax_par + 1)), kper = (int)(i2 / (kmax_par + 1));
+    double a = 0.0;
+    if (fixk) {
+        const int kzb = kpar / NL, r = kpar % NL;
+        for (int c = 0; c < nchunk; ++c) {
+            const long long b = (long long)kzb * nchunk + c;
+            if (b < nblocks) a += part[b * RS + n31 + r * nper + kper];
+        }
+    } else {
+        const int us[2] = {kpar, N - kpar};
+        const int nu = (kpar > 0 && N - kpar != kpar && N - kpar > N / 2) ? 2 : 1;
+        for (int q = 0; q < nu; ++q)
+            for (int c = 0; c < nchunk; ++c) a += part[((long long)us[q] * nchunk + c) * RS + n31 + kper];
+    }
+    if (a != 0.0) g2[i2] += a;
 }
 
 static int num_cus() {
@@ -404,28 +547,41 @@ static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int ax
                            const PkSums& sums, hipStream_t st) {
     constexpr int NL = nl_bin<R, N>();
     constexpr int T = threads_bin<N, NL>();
-    const PkDims d = pk_dims(N, P);
+    const int W = N / 2 + 1;   // kz = 0 .. N/2; rows of the spectrum may be padded to P >= W
+    const PkDims d = pk_dims(N, W);
     const cplx<R>* tw = twiddles<R>(N);
     const double* win = window_table(N, p);
     PYL_CHECK(tw && win, E_ALLOC, "table allocation failed");
     const int last = pk_last_axis(axis);
     const int fixk = axis == 2 ? 1 : 0;
-    const int nkzb = (int)((P + NL - 1) / NL);
+    const int nkzb = (W + NL - 1) / NL;
     // about two blocks per CU, all resident at once; each walks its column blocks with prefetch
     const int want = 2 * num_cus();
     int nchunk, grid;
     if (fixk) {
-        nchunk = std::min(N, std::max(1, (want + nkzb - 1) / nkzb));
+        nchunk = std::min(N, std::max(1, want / nkzb));
         grid = nkzb * nchunk;
     } else {
-        nchunk = std::min(nkzb, std::max(1, (want + N - 1) / N));
+        nchunk = std::min(nkzb, std::max(1, want / N));
         grid = N * nchunk;
     }
     const int nper = d.kmax_per + 1;
-    size_t shm = sizeof(double) * (4 * (size_t)d.n3() + d.n1() + (size_t)(fixk ? NL : 1) * nper);
-    hipLaunchKernelGGL((k_last_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P,
-                       last, axis, fixk, nchunk, flags, d.kmax_par, d.kmax_per, d.n1(), d.n3(), win, tw, sums.s3,
-                       sums.s1, sums.s2);
+    const int RS = 4 * d.n3() + d.n1() + (fixk ? NL : 1) * nper;
+    size_t shm = sizeof(double) * (size_t)RS;
+    int rc = OK;
+    double* part = (double*)workspace(WS_PART, sizeof(double) * (size_t)grid * RS, &rc);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_last_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P, W,
+                       last, axis, fixk, nchunk, flags, d.kmax_par, d.kmax_per, d.n1(), d.n3(), win, tw, part, RS);
+    PYL_HIP(hipGetLastError());
+    const int n31 = 4 * d.n3() + d.n1();
+    const int slices = std::max(1, std::min(16, grid / 32));
+    hipLaunchKernelGGL(k_bin_reduce31, dim3((unsigned)((n31 + 63) / 64), (unsigned)slices), dim3(256), 0, st, part, RS,
+                       grid, d.n3(), d.n1(), sums.s3, sums.s1);
+    PYL_HIP(hipGetLastError());
+    const long long n2 = (long long)(d.kmax_par + 1) * nper;
+    hipLaunchKernelGGL(k_bin_reduce2, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, part, RS, grid, n31, nper,
+                       d.kmax_par, fixk, nchunk, NL, N, sums.s2);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -479,7 +635,8 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
     double* kper = kpar + n2;
     double* P2D = kper + n2;
     double* N2 = P2D + n2;
-    for (int i = threadIdx.x; i < kmax; i += blockDim.x) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    for (int i = gi; i < kmax; i += gs) {
         const double n = (double)n3c[i + 1];
         k3D[i] = (k3s[i + 1] / n) * kF;
         P0[i] = (s3[4 * (i + 1) + 0] / n) * unit;
@@ -488,7 +645,7 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
         ph[i] = (s3[4 * (i + 1) + 3] / n) * unit;
         N3[i] = n;
     }
-    for (int i = threadIdx.x; i < kmax_par; i += blockDim.x) {
+    for (int i = gi; i < kmax_par; i += gs) {
         const double n = (double)n1c[i + 1];
         const double k = ((double)(i + 1) * n / n) * kF;
         const double kmaxper = sqrt(kN * kN - k * k);
@@ -496,7 +653,7 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
         P1[i] = s1[i + 1] * unit * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
         N1[i] = n;
     }
-    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+    for (int i = gi; i < n2; i += gs) {
         const int kp = i % (kmax_par + 1), kq = i / (kmax_par + 1);
         const double n = (double)n2c[i];
         kpar[i] = 0.5 * (kp + kp + 1) * kF;
@@ -508,7 +665,9 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
 
 int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, double* out, hipStream_t st) {
     const PkDims& d = g->d;
-    hipLaunchKernelGGL(k_pk_finalize, dim3(1), dim3(1024), 0, st, d.N, d.kmax, d.kmax_par, d.n2(), BoxSize, g->n1,
+    const int m = std::max(d.kmax, d.n2());
+    hipLaunchKernelGGL(k_pk_finalize, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, d.N, d.kmax, d.kmax_par,
+                       d.n2(), BoxSize, g->n1,
                        g->n2, g->n3, g->k3sum, sums.s3, sums.s1, sums.s2, out);
     PYL_HIP(hipGetLastError());
     return OK;
