@@ -22,6 +22,22 @@ import numpy as np
 
 METRIC = "Mparticles/s CIC+Pk end-to-end at 512^3; FP32 vs FP64 speedup at 1e-5 tol"
 HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
+# HBM bytes per launch of each stage's kernels at this workload, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes scaled by what the counters read for a plain device copy of known size
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_zeldovich512_cic_f32.txt")
+
+
+def measured_traffic(workload):
+    """{stage: bytes per launch} from the committed PMC summary (lines 'workload stage bytes')."""
+    out = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return out
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 3 and not line.startswith("#") and f[0] == workload:
+            out[f[1]] = int(float(f[2]))
+    return out
 
 
 def algorithmic_bytes(n_p, N, S=2, tile=16):
@@ -195,6 +211,9 @@ def main():
     ap.add_argument("--deposit", default=None, choices=[None, "atomic", "binned"])
     ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 oracle check")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="before the timed region, copy a 512 MiB device buffer twice (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--workload", default="zeldovich512", choices=["zeldovich512", "batch", "adjoint", "bispectrum"])
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--realizations", type=int, default=16)
@@ -214,6 +233,13 @@ def main():
     from pylians_amd.pipeline import run_device
 
     _lib.require_gpu()
+    if args.calibrate_copy:
+        src = torch.ones(128 << 20, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n_side, N, L, axis = args.n_side, args.grid, 1000.0, 0
     scheme = _lib.SCHEMES[args.mas]
     t0 = time.time()
@@ -272,9 +298,12 @@ def main():
     per = {k: (v[1] / v[0]) for k, v in stages.items() if v[0] > 0}
     dom = max((k for k in per if k in ab), key=lambda k: per[k])
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
+    workload = "zeldovich%d_rsd_%s_pk_grid%d" % (n_side, args.mas.lower(), N)
+    traffic = measured_traffic(workload) if args.deposit is None else {}
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": ab[dom], "avg_ms": round(per[dom], 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
+                "bytes_per_launch": ab[dom], "avg_ms": round(per[dom], 4),
+                "traffic_by_stage": traffic or None}
     stage_ms = {k: round(v, 4) for k, v in per.items()}
 
     line = {
@@ -282,7 +311,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (Zel'dovich lattice + RSD, seeded, generated on the GPU)",
-        "config": {"workload": "zeldovich%d_rsd_%s_pk_grid%d" % (n_side, args.mas.lower(), N), "n_particles": n_p,
+        "config": {"workload": workload, "n_particles": n_p,
                    "grid": N, "BoxSize": L, "MAS": args.mas, "axis": axis, "outputs": "P0/P2/P4, 1-D, 2-D, phase",
                    "deposit": args.deposit or "auto"},
         "fp64_ms_per_step": round(ms64, 4), "speedup_fp32c_vs_fp64": round(ms64 / ms, 3),
