@@ -43,23 +43,18 @@ def measured_traffic(workload):
 def algorithmic_bytes(n_p, N, S=2, tile=16):
     """Per-kernel bytes each launch must move: inputs read once, outputs written once.
 
-    Float32 path, no weights.  The cell-binned deposit writes 12-byte particle records into tile
-    buckets (k_bin), reads them back once and writes the K^3 interior cells of each 16^3 tile plus
-    float64 partials of its BSZ boundary cells (k_tile), and the merge reads those partials and
-    writes the SHELL cells each tile owns (k_merge); K = 16 - (S - 1).
+    Float32 path, no weights.  The cell-binned deposit writes one 12-byte record per particle into
+    its (tile, halo class) bucket segment (k_bin); each 16^3 tile then reads its own records and the
+    records of its lower neighbours whose stencils reach it ((1 + (S-1)/16)^3 records per particle
+    on average) and writes its cells once (k_tile).
     """
     Ng = N ** 3
     Nh = N * N * (N // 2 + 1)
-    H = S - 1
-    M, K = tile + H, tile - H
-    bsz = 2 * H * M * M + K * 2 * H * M + K * K * 2 * H
-    shell = tile ** 3 - K ** 3
-    nt = (N // tile) ** 3
+    halo = (1.0 + (S - 1) / tile) ** 3
     return {
         "deposit_atomic": 12 * n_p + 4 * Ng,     # read positions, add into the float32 grid
         "deposit_bin": 24 * n_p,                 # read positions, write bucket records
-        "deposit_tiles": 12 * n_p + nt * (4 * K ** 3 + 8 * bsz),
-        "deposit_merge": nt * (8 * bsz + 4 * shell),
+        "deposit_tiles": int(12 * n_p * halo) + 4 * Ng,
         "grid_zero": 4 * Ng,
         "fft_z_r2c": 4 * Ng + 8 * Nh,            # read the real grid, write the half spectrum
         "fft_mid": 16 * Nh,                       # read + write the half spectrum

@@ -1,17 +1,18 @@
-// Cell-binned mass assignment for gfx950: no global atomics on the grid.
+// Cell-binned mass assignment for gfx950: no global atomics on the grid, every cell written once.
 //
-// 1. k_bin      particles -> fixed-capacity buckets, one per TILE^3 block of cells, keyed by the
-//               tile holding the particle's lowest stencil cell (wave-aggregated bucket counters).
-//               Particles beyond a bucket's capacity go to an overflow list.
-// 2. k_ov_*     the overflow list is grouped by tile with an exact count + scan (empty in the
-//               common case: the kernels then only read a zero counter).
-// 3. k_tile     one workgroup per tile accumulates all stencil weights of its particles into an
-//               LDS block of (TILE+S-1)^3 float64 cells (ds_add_f64: the per-cell sum is exact to
-//               float64, the "compensated" accumulator), then writes the cells no other tile can
-//               touch straight to the grid (one rounding) and hands the boundary cells (low S-1
-//               layers and the S-1 spill layers beyond the high faces) to a float64 buffer.
-// 4. k_merge    every tile sums the float64 partials of its low S-1 layers from itself and its up
-//               to 7 lower neighbours and writes them (one rounding).
+// The grid is cut into TILE^3 tiles.  A particle belongs to the tile of its lowest stencil cell; its
+// halo class is the 3-bit mask of the axes along which its stencil reaches into the next tile (local
+// lowest cell > TILE - S).  Buckets are segmented by (tile, class).
+// 1. k_bin    particles -> bucket segments.  A workgroup takes 512 consecutive particles at a time,
+//             counts them per (tile, class) key in an LDS hash table, reserves room with one global
+//             atomic per distinct key and writes the records (position, and weight).  Records beyond
+//             a segment's capacity go to an overflow list.
+// 2. k_ov_*   the overflow list is grouped by tile with an exact count + scan (empty in the common
+//             case: the kernels then only read a zero counter).
+// 3. k_tile   one workgroup per tile accumulates into TILE^3 float64 LDS cells (ds_add_f64: the per-
+//             cell sum is exact to float64, the "compensated" accumulator) the stencil weights that
+//             fall inside the tile, from its own particles and from those of its 7 lower neighbours
+//             whose class reaches it, then writes every cell once with one rounding.
 // Output modes: accumulate into the existing grid (MAS_library.MA semantics), overwrite, or the
 // overdensity delta = rho * a - 1 (a = N^3 / mass) evaluated in float64 before the single rounding.
 #include <math.h>
@@ -23,6 +24,8 @@
 namespace pyl {
 
 constexpr int TILE = 16;
+constexpr int BT = 512;    // k_bin: threads per workgroup = particles per step
+constexpr int HT = 1024;   // k_bin: LDS hash slots (> BT, so an insertion always finds a slot)
 
 // Lowest stencil cell (unwrapped) and the float64 1-D weights; the same arithmetic as the
 // stencils of deposit.hip (MAS_library.pyx:151-157, 290, 393-399, 486-492).
@@ -71,78 +74,45 @@ __device__ __forceinline__ int wrapN(long long b, int N) {
     return (int)(r < 0 ? r + N : r);
 }
 
-template <int S> struct Layout {
-    static constexpr int M = TILE + S - 1;  // extended block edge
-    static constexpr int H = S - 1;         // halo thickness
-    static constexpr int K = TILE - H;      // interior edge
-    static constexpr int XSZ = 2 * H * M * M;
-    static constexpr int YSZ = K * 2 * H * M;
-    static constexpr int ZSZ = K * K * 2 * H;
-    static constexpr int BSZ = XSZ + YSZ + ZSZ;  // boundary cells per tile
-    static constexpr int SHELL = TILE * TILE * TILE - K * K * K;
-    // compact index of a boundary cell (i, j, k) of the extended block
-    static __device__ __forceinline__ int bidx(int i, int j, int k) {
-        if (i < H || i >= TILE) {
-            const int ii = i < H ? i : i - TILE + H;
-            return (ii * M + j) * M + k;
-        }
-        const int i2 = i - H;
-        if (j < H || j >= TILE) {
-            const int jj = j < H ? j : j - TILE + H;
-            return XSZ + (i2 * 2 * H + jj) * M + k;
-        }
-        const int j2 = j - H;
-        const int kk = k < H ? k : k - TILE + H;
-        return XSZ + YSZ + (i2 * K + j2) * 2 * H + kk;
-    }
-    static __device__ __forceinline__ void bcell(int idx, int& i, int& j, int& k) {
-        if (idx < XSZ) {
-            const int ii = idx / (M * M), r = idx % (M * M);
-            i = ii < H ? ii : ii - H + TILE;
-            j = r / M;
-            k = r % M;
-        } else if (idx < XSZ + YSZ) {
-            const int r = idx - XSZ, i2 = r / (2 * H * M), q = r % (2 * H * M), jj = q / M;
-            i = i2 + H;
-            j = jj < H ? jj : jj - H + TILE;
-            k = q % M;
-        } else {
-            const int r = idx - XSZ - YSZ, i2 = r / (K * 2 * H), q = r % (K * 2 * H), j2 = q / (2 * H),
-                      kk = q % (2 * H);
-            i = i2 + H;
-            j = j2 + H;
-            k = kk < H ? kk : kk - H + TILE;
-        }
-    }
-};
-
 struct BinArgs {
     const float* pos;
     const float* W;
     long long n;
     int N, nt;           // grid edge, tiles per edge
     double inv;          // N / BoxSize
-    unsigned* fill;      // [ntiles]
-    long long C;         // bucket capacity (records)
-    float* bucket;       // [ntiles * C * R]
+    unsigned* fill;      // [ntiles * 8] records per (tile, class)
+    float* bucket;       // [ntiles * CT * R]
+    long long CT;        // record slots per tile
+    int cofs[8], cap[8]; // first slot and capacity of each class segment
     unsigned* ov_count;  // [1]
     long long* ov_idx;   // [n]
-    unsigned* ov_tile;   // [n]
+    unsigned* ov_key;    // [n]
 };
 
+template <int S>
+__device__ __forceinline__ int halo_class(int bx, int by, int bz) {
+    return ((bx % TILE) > TILE - S ? 1 : 0) | ((by % TILE) > TILE - S ? 2 : 0) | ((bz % TILE) > TILE - S ? 4 : 0);
+}
+
 template <int S, bool HASW>
-__global__ __launch_bounds__(256) void k_bin(BinArgs a) {
+__global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
     constexpr int R = HASW ? 4 : 3;
-    const int lane = threadIdx.x & 63;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long first = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    // every lane of a wave runs the same number of iterations (ballots below need them all)
-    const long long iters = (a.n + stride - 1) / stride;
-    for (long long it = 0; it < iters; ++it) {
-        const long long p = first + it * stride;
+    __shared__ int hkey[HT];
+    __shared__ unsigned hcnt[HT];
+    __shared__ unsigned hbase[HT];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < HT; i += BT) {
+        hkey[i] = -1;
+        hcnt[i] = 0u;
+    }
+    __syncthreads();
+    const long long nstep = (a.n + BT - 1) / BT;
+    for (long long step = blockIdx.x; step < nstep; step += gridDim.x) {
+        const long long p = step * BT + tid;
         const bool valid = p < a.n;
         float x = 0.f, y = 0.f, z = 0.f, w = 0.f;
-        int t = -1;
+        int key = -1, slot = 0;
+        unsigned rank = 0u;
         if (valid) {
             x = a.pos[3 * p];
             y = a.pos[3 * p + 1];
@@ -152,49 +122,52 @@ __global__ __launch_bounds__(256) void k_bin(BinArgs a) {
             const int bx = wrapN(Axis<S>::eval((double)x * a.inv, ww), a.N);
             const int by = wrapN(Axis<S>::eval((double)y * a.inv, ww), a.N);
             const int bz = wrapN(Axis<S>::eval((double)z * a.inv, ww), a.N);
-            t = ((bx / TILE) * a.nt + by / TILE) * a.nt + bz / TILE;
+            key = (((bx / TILE) * a.nt + by / TILE) * a.nt + bz / TILE) * 8 + halo_class<S>(bx, by, bz);
+            // at most BT distinct keys per step in HT > BT slots: the probe always ends
+            slot = (int)(((unsigned)key * 2654435761u) >> 22) & (HT - 1);
+            for (;;) {
+                const int prev = atomicCAS(&hkey[slot], -1, key);
+                if (prev == -1 || prev == key) break;
+                slot = (slot + 1) & (HT - 1);
+            }
+            rank = atomicAdd(&hcnt[slot], 1u);
         }
-        unsigned long long active = __ballot(valid);
-        long long slot = -1;
-        while (active) {
-            const int leader = __ffsll((long long)active) - 1;
-            const int key = __shfl(t, leader, 64);
-            const unsigned long long m = __ballot(t == key) & active;
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(&a.fill[key], (unsigned)__popcll(m));
-            base = __shfl(base, leader, 64);
-            if ((m >> lane) & 1ull) slot = (long long)base + __popcll(m & ((1ull << lane) - 1ull));
-            active &= ~m;
+        __syncthreads();
+        for (int i = tid; i < HT; i += BT) {
+            const int k = hkey[i];
+            if (k >= 0) hbase[i] = atomicAdd(&a.fill[k], hcnt[i]);
         }
-        const bool inb = valid && slot < a.C;
-        if (inb) {
-            float* r = a.bucket + ((long long)t * a.C + slot) * R;
-            r[0] = x;
-            r[1] = y;
-            r[2] = z;
-            if (HASW) r[3] = w;
-        }
-        const unsigned long long ovm = __ballot(valid && !inb);
-        if (ovm) {
-            const int leader = __ffsll((long long)ovm) - 1;
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(a.ov_count, (unsigned)__popcll(ovm));
-            base = __shfl(base, leader, 64);
-            if ((ovm >> lane) & 1ull) {
-                const unsigned o = base + __popcll(ovm & ((1ull << lane) - 1ull));
+        __syncthreads();
+        if (valid) {
+            const int c = key & 7;
+            const unsigned idx = hbase[slot] + rank;
+            if (idx < (unsigned)a.cap[c]) {
+                float* r = a.bucket + ((long long)(key >> 3) * a.CT + a.cofs[c] + idx) * R;
+                r[0] = x;
+                r[1] = y;
+                r[2] = z;
+                if (HASW) r[3] = w;
+            } else {
+                const unsigned o = atomicAdd(a.ov_count, 1u);
                 a.ov_idx[o] = p;
-                a.ov_tile[o] = (unsigned)t;
+                a.ov_key[o] = (unsigned)key;
             }
         }
+        __syncthreads();
+        for (int i = tid; i < HT; i += BT) {
+            hkey[i] = -1;
+            hcnt[i] = 0u;
+        }
+        __syncthreads();
     }
 }
 
 // overflow grouping: count per tile, exclusive scan, scatter (records copied from the inputs)
 __global__ __launch_bounds__(256) void k_ov_count(const unsigned* __restrict__ ov_count,
-                                                   const unsigned* __restrict__ ov_tile, unsigned* __restrict__ ocnt) {
+                                                   const unsigned* __restrict__ ov_key, unsigned* __restrict__ ocnt) {
     const long long n = *ov_count;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        atomicAdd(&ocnt[ov_tile[i]], 1u);
+        atomicAdd(&ocnt[ov_key[i] >> 3], 1u);
 }
 
 __global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ off, int n) {
@@ -223,15 +196,15 @@ __global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ cnt,
 template <bool HASW>
 __global__ __launch_bounds__(256) void k_ov_scatter(const unsigned* __restrict__ ov_count,
                                                      const long long* __restrict__ ov_idx,
-                                                     const unsigned* __restrict__ ov_tile,
+                                                     const unsigned* __restrict__ ov_key,
                                                      const unsigned* __restrict__ ooff, unsigned* __restrict__ ocur,
                                                      const float* __restrict__ pos, const float* __restrict__ W,
-                                                     float* __restrict__ grouped) {
+                                                     float* __restrict__ grouped, unsigned* __restrict__ gcls) {
     constexpr int R = HASW ? 4 : 3;
     const long long n = *ov_count;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        const unsigned t = ov_tile[i];
+        const unsigned key = ov_key[i], t = key >> 3;
         const long long p = ov_idx[i];
         const unsigned s = ooff[t] + atomicAdd(&ocur[t], 1u);
         float* r = grouped + (long long)s * R;
@@ -239,6 +212,7 @@ __global__ __launch_bounds__(256) void k_ov_scatter(const unsigned* __restrict__
         r[1] = pos[3 * p + 1];
         r[2] = pos[3 * p + 2];
         if (HASW) r[3] = W[p];
+        gcls[s] = key & 7u;
     }
 }
 
@@ -257,114 +231,115 @@ __device__ __forceinline__ void put(G* __restrict__ grid, long long g, double rh
 struct TileArgs {
     const float* bucket;
     const unsigned* fill;
-    long long C;
+    long long CT;
+    int cofs[8], cap[8];
     const float* grouped;
+    const unsigned* gcls;
     const unsigned* ooff;
     const unsigned* ocnt;
     int N, nt;
     double inv;
     double mult;          // weight multiplier
     void* grid;
-    double* bnd;          // [ntiles * BSZ]
     const double* dmass;  // device mass (nullable) for OUT_DELTA
     double mass;
 };
 
+// the stencil weights of one particle that fall inside the tile with lowest cell (ox, oy, oz)
 template <int S, bool HASW>
-__device__ __forceinline__ void accumulate(double* acc, const float* r, int N, int tx, int ty, int tz, double inv,
+__device__ __forceinline__ void accumulate(double* acc, const float* r, int N, int ox, int oy, int oz, double inv,
                                            double mult) {
-    constexpr int M = Layout<S>::M;
     double w0[S], w1[S], w2[S];
-    const int l0 = wrapN(Axis<S>::eval((double)r[0] * inv, w0), N) - tx * TILE;
-    const int l1 = wrapN(Axis<S>::eval((double)r[1] * inv, w1), N) - ty * TILE;
-    const int l2 = wrapN(Axis<S>::eval((double)r[2] * inv, w2), N) - tz * TILE;
+    int l0 = wrapN(Axis<S>::eval((double)r[0] * inv, w0), N) - ox;
+    int l1 = wrapN(Axis<S>::eval((double)r[1] * inv, w1), N) - oy;
+    int l2 = wrapN(Axis<S>::eval((double)r[2] * inv, w2), N) - oz;
+    // a particle of the lower neighbour across the periodic boundary
+    if (l0 >= TILE) l0 -= N;
+    if (l1 >= TILE) l1 -= N;
+    if (l2 >= TILE) l2 -= N;
     const double ww = HASW ? (double)r[3] * mult : mult;
 #pragma unroll
-    for (int l = 0; l < S; ++l)
+    for (int l = 0; l < S; ++l) {
+        const int c0 = l0 + l;
+        if (c0 < 0 || c0 >= TILE) continue;
 #pragma unroll
         for (int m = 0; m < S; ++m) {
+            const int c1 = l1 + m;
+            if (c1 < 0 || c1 >= TILE) continue;
             const double wlm = w0[l] * w1[m] * ww;
-            double* row = acc + ((l0 + l) * M + (l1 + m)) * M + l2;
+            double* row = acc + (c0 * TILE + c1) * TILE;
 #pragma unroll
-            for (int q = 0; q < S; ++q) atomicAdd(row + q, wlm * w2[q]);
+            for (int q = 0; q < S; ++q) {
+                const int c2 = l2 + q;
+                if (c2 >= 0 && c2 < TILE) atomicAdd(row + c2, wlm * w2[q]);
+            }
         }
+    }
 }
 
 template <int S, typename G, bool HASW, int MODE>
 __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
-    using Lt = Layout<S>;
-    constexpr int M = Lt::M, H = Lt::H, K = Lt::K;
     constexpr int R = HASW ? 4 : 3;
-    __shared__ double acc[M * M * M];
-    const int t = blockIdx.x;
-    const int tx = t / (a.nt * a.nt), ty = (t / a.nt) % a.nt, tz = t % a.nt;
-    for (int i = threadIdx.x; i < M * M * M; i += 256) acc[i] = 0.0;
-    __syncthreads();
-    const long long cnt = min((long long)a.fill[t], a.C);
-    const float* rec = a.bucket + (long long)t * a.C * R;
-    for (long long p = threadIdx.x; p < cnt; p += 256)
-        accumulate<S, HASW>(acc, rec + p * R, a.N, tx, ty, tz, a.inv, a.mult);
-    const long long o0 = a.ooff[t], oc = a.ocnt[t];
-    for (long long p = threadIdx.x; p < oc; p += 256)
-        accumulate<S, HASW>(acc, a.grouped + (o0 + p) * R, a.N, tx, ty, tz, a.inv, a.mult);
-    __syncthreads();
-    G* grid = (G*)a.grid;
-    const double am = (MODE == OUT_DELTA) ? (double)a.N * a.N * a.N / (a.dmass ? a.dmass[0] : a.mass) : 0.0;
-    const long long N = a.N;
-    // cells only this tile touches: local [H, TILE)^3
-    for (int idx = threadIdx.x; idx < K * K * K; idx += 256) {
-        const int i = H + idx / (K * K), j = H + (idx / K) % K, k = H + idx % K;
-        const long long g = (((long long)tx * TILE + i) * N + ty * TILE + j) * N + tz * TILE + k;
-        put<G, MODE>(grid, g, acc[(i * M + j) * M + k], am);
-    }
-    if (H > 0) {
-        double* b = a.bnd + (long long)t * Lt::BSZ;
-        for (int idx = threadIdx.x; idx < Lt::BSZ; idx += 256) {
-            int i, j, k;
-            Lt::bcell(idx, i, j, k);
-            b[idx] = acc[(i * M + j) * M + k];
-        }
-    }
-}
-
-template <int S, typename G, int MODE>
-__global__ __launch_bounds__(256) void k_merge(TileArgs a) {
-    using Lt = Layout<S>;
-    constexpr int H = Lt::H, K = Lt::K;
-    const int t = blockIdx.x;
-    const int nt = a.nt;
+    constexpr int NE = 72;   // 8 neighbours (d = 0 own) x (8 class segments + 1 overflow group)
+    __shared__ double acc[TILE * TILE * TILE];
+    __shared__ long long sbase[NE];
+    __shared__ unsigned scnt[NE];
+    __shared__ unsigned sstart[NE + 1];
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x, nt = a.nt;
     const int tx = t / (nt * nt), ty = (t / nt) % nt, tz = t % nt;
+    for (int i = tid; i < TILE * TILE * TILE; i += 256) acc[i] = 0.0;
+    // entry e = d * 9 + c: class segment c (< 8) or the overflow group (c = 8) of the neighbour
+    // t - d; a neighbour's particles reach this tile when their class contains d
+    if (tid < NE) {
+        const int d = tid / 9, c = tid % 9;
+        const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+        const long long tn = ((long long)((tx - dx + nt) % nt) * nt + (ty - dy + nt) % nt) * nt + (tz - dz + nt) % nt;
+        unsigned n = 0u;
+        long long b = 0;
+        if (c < 8) {
+            if ((c & d) == d) {
+                n = min(a.fill[tn * 8 + c], (unsigned)a.cap[c]);
+                b = tn * a.CT + a.cofs[c];
+            }
+        } else {
+            n = a.ocnt[tn];
+            b = a.ooff[tn];
+        }
+        scnt[tid] = n;
+        sbase[tid] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned run = 0u;
+        for (int e = 0; e < NE; ++e) {
+            sstart[e] = run;
+            run += scnt[e];
+        }
+        sstart[NE] = run;
+    }
+    __syncthreads();
+    const unsigned tot = sstart[NE];
+    const int ox = tx * TILE, oy = ty * TILE, oz = tz * TILE;
+    int e = 0;
+    for (unsigned f = tid; f < tot; f += 256) {
+        while (f >= sstart[e + 1]) ++e;
+        const long long i = sbase[e] + (f - sstart[e]);
+        if (e % 9 < 8) {
+            accumulate<S, HASW>(acc, a.bucket + i * R, a.N, ox, oy, oz, a.inv, a.mult);
+        } else {
+            const unsigned d = (unsigned)(e / 9);
+            if ((a.gcls[i] & d) == d) accumulate<S, HASW>(acc, a.grouped + i * R, a.N, ox, oy, oz, a.inv, a.mult);
+        }
+    }
+    __syncthreads();
     G* grid = (G*)a.grid;
     const double am = (MODE == OUT_DELTA) ? (double)a.N * a.N * a.N / (a.dmass ? a.dmass[0] : a.mass) : 0.0;
     const long long N = a.N;
-    constexpr int XL = H * TILE * TILE, YL = K * H * TILE;
-    for (int idx = threadIdx.x; idx < Lt::SHELL; idx += 256) {
-        int i, j, k;
-        if (idx < XL) {
-            i = idx / (TILE * TILE);
-            j = (idx / TILE) % TILE;
-            k = idx % TILE;
-        } else if (idx < XL + YL) {
-            const int r = idx - XL;
-            i = H + r / (H * TILE);
-            j = (r / TILE) % H;
-            k = r % TILE;
-        } else {
-            const int r = idx - XL - YL;
-            i = H + r / (K * H);
-            j = H + (r / H) % K;
-            k = r % H;
-        }
-        double s = 0.0;
-        for (int dx = 0; dx <= (i < H ? 1 : 0); ++dx)
-            for (int dy = 0; dy <= (j < H ? 1 : 0); ++dy)
-                for (int dz = 0; dz <= (k < H ? 1 : 0); ++dz) {
-                    const int sx = (tx - dx + nt) % nt, sy = (ty - dy + nt) % nt, sz = (tz - dz + nt) % nt;
-                    const long long st = ((long long)sx * nt + sy) * nt + sz;
-                    s += a.bnd[st * Lt::BSZ + Lt::bidx(i + dx * TILE, j + dy * TILE, k + dz * TILE)];
-                }
-        const long long g = (((long long)tx * TILE + i) * N + ty * TILE + j) * N + tz * TILE + k;
-        put<G, MODE>(grid, g, s, am);
+    for (int idx = tid; idx < TILE * TILE * TILE; idx += 256) {
+        const int i = idx / (TILE * TILE), j = (idx / TILE) % TILE, k = idx % TILE;
+        const long long g = (((long long)ox + i) * N + oy + j) * N + oz + k;
+        put<G, MODE>(grid, g, acc[idx], am);
     }
 }
 
@@ -374,59 +349,69 @@ bool binned_supported(int axes, int dims) { return axes == 3 && dims % TILE == 0
 template <int S, typename G, bool HASW, int MODE>
 static int run_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
                       const double* dmass, double mass, hipStream_t st) {
-    using Lt = Layout<S>;
     constexpr int R = HASW ? 4 : 3;
     int rc = OK;
     const int nt = dims / TILE;
     const long long ntiles = (long long)nt * nt * nt;
-    const long long avg = (n + ntiles - 1) / ntiles;
-    const long long C = ((2 * avg + 64) + 15) / 16 * 16;
-    // counters: fill | ocnt | ocur | ooff (ntiles+1) | ov_count
-    const size_t ncnt = 4 * (size_t)ntiles + 2;
+    // class segment capacities: twice the expected count of a uniform distribution plus slack
+    const double avg = (double)n / (double)ntiles;
+    const double ph = (double)(S - 1) / TILE, p0 = 1.0 - ph;
+    int cap[8], cofs[8];
+    long long CT = 0;
+    for (int c = 0; c < 8; ++c) {
+        const double f = ((c & 1) ? ph : p0) * ((c & 2) ? ph : p0) * ((c & 4) ? ph : p0);
+        const long long v = f > 0.0 ? ((long long)(2.0 * avg * f) + 32 + 15) / 16 * 16 : 0;
+        PYL_CHECK(v < (1LL << 30), E_UNSUPPORTED, "binned deposit: too many particles per tile");
+        cap[c] = (int)v;
+        cofs[c] = (int)CT;
+        CT += v;
+    }
+    // counters: fill (8 * ntiles) | ocnt | ocur | ooff (ntiles + 1) | ov_count
+    const size_t ncnt = 11 * (size_t)ntiles + 2;
     unsigned* cnt = (unsigned*)workspace(WS_BINS, sizeof(unsigned) * ncnt + 64, &rc);
     if (rc) return rc;
-    unsigned *fill = cnt, *ocnt = cnt + ntiles, *ocur = cnt + 2 * ntiles, *ooff = cnt + 3 * ntiles,
-             *ov_count = cnt + 4 * ntiles + 1;
-    float* bucket = (float*)workspace(WS_AUX, sizeof(float) * (size_t)ntiles * C * R, &rc);
+    unsigned *fill = cnt, *ocnt = cnt + 8 * ntiles, *ocur = cnt + 9 * ntiles, *ooff = cnt + 10 * ntiles,
+             *ov_count = cnt + 11 * ntiles + 1;
+    float* bucket = (float*)workspace(WS_AUX, sizeof(float) * (size_t)ntiles * CT * R, &rc);
     if (rc) return rc;
-    char* ov = (char*)workspace(WS_AUX2, (size_t)n * (8 + 4 + 4 * R) + 256, &rc);
+    // overflow: ov_idx (8n) | ov_key (4n) | gcls (4n) | grouped (4Rn)
+    char* ov = (char*)workspace(WS_AUX2, (size_t)n * (16 + 4 * R) + 256, &rc);
     if (rc) return rc;
     long long* ov_idx = (long long*)ov;
-    unsigned* ov_tile = (unsigned*)(ov + 8 * (size_t)n);
-    float* grouped = (float*)(ov + 12 * (size_t)n + 64);
-    double* bnd = nullptr;
-    if (Lt::BSZ > 0) {
-        bnd = (double*)workspace(WS_AUX3, sizeof(double) * (size_t)ntiles * Lt::BSZ, &rc);
-        if (rc) return rc;
-    }
+    unsigned* ov_key = (unsigned*)(ov + 8 * (size_t)n);
+    unsigned* gcls = (unsigned*)(ov + 12 * (size_t)n);
+    float* grouped = (float*)(ov + 16 * (size_t)n + 64);
     rc = zero_fill(cnt, sizeof(unsigned) * ncnt, st);
     if (rc) return rc;
     {
         StageTimer tm("deposit_bin", st);
-        BinArgs ba{pos, W, (long long)n, dims, nt, dims / BoxSize, fill, C, bucket, ov_count, ov_idx, ov_tile};
-        long long blocks = (n + 255) / 256;
-        if (blocks > 8192) blocks = 8192;
-        if (blocks < 1) blocks = 1;
-        hipLaunchKernelGGL((k_bin<S, HASW>), dim3((unsigned)blocks), dim3(256), 0, st, ba);
+        BinArgs ba{pos, W, (long long)n, dims, nt, dims / BoxSize, fill, bucket, CT, {}, {}, ov_count, ov_idx, ov_key};
+        for (int c = 0; c < 8; ++c) {
+            ba.cofs[c] = cofs[c];
+            ba.cap[c] = cap[c];
+        }
+        long long blocks = (n + BT - 1) / BT;
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL((k_bin<S, HASW>), dim3((unsigned)blocks), dim3(BT), 0, st, ba);
         PYL_HIP(hipGetLastError());
     }
     {
         StageTimer tm("deposit_overflow", st);
-        hipLaunchKernelGGL(k_ov_count, dim3(256), dim3(256), 0, st, ov_count, ov_tile, ocnt);
+        hipLaunchKernelGGL(k_ov_count, dim3(256), dim3(256), 0, st, ov_count, ov_key, ocnt);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, ocnt, ooff, (int)ntiles);
-        hipLaunchKernelGGL((k_ov_scatter<HASW>), dim3(256), dim3(256), 0, st, ov_count, ov_idx, ov_tile, ooff, ocur,
-                           pos, W, grouped);
+        hipLaunchKernelGGL((k_ov_scatter<HASW>), dim3(256), dim3(256), 0, st, ov_count, ov_idx, ov_key, ooff, ocur,
+                           pos, W, grouped, gcls);
         PYL_HIP(hipGetLastError());
     }
-    TileArgs ta{bucket, fill, C, grouped, ooff, ocnt, dims, nt, dims / BoxSize, mult, grid, bnd, dmass, mass};
     {
         StageTimer tm("deposit_tiles", st);
+        TileArgs ta{bucket, fill, CT, {}, {}, grouped, gcls, ooff, ocnt, dims, nt, dims / BoxSize, mult, grid, dmass,
+                    mass};
+        for (int c = 0; c < 8; ++c) {
+            ta.cofs[c] = cofs[c];
+            ta.cap[c] = cap[c];
+        }
         hipLaunchKernelGGL((k_tile<S, G, HASW, MODE>), dim3((unsigned)ntiles), dim3(256), 0, st, ta);
-        PYL_HIP(hipGetLastError());
-    }
-    if (Lt::BSZ > 0) {
-        StageTimer tm("deposit_merge", st);
-        hipLaunchKernelGGL((k_merge<S, G, MODE>), dim3((unsigned)ntiles), dim3(256), 0, st, ta);
         PYL_HIP(hipGetLastError());
     }
     return OK;
