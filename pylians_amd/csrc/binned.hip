@@ -3,9 +3,10 @@
 // The grid is cut into TILE^3 tiles.  A particle belongs to the tile of its lowest stencil cell; its
 // halo class is the 3-bit mask of the axes along which its stencil reaches into the next tile (local
 // lowest cell > TILE - S).  Buckets are segmented by (tile, class).
-// 1. k_bin    particles -> bucket segments.  A workgroup takes 512 consecutive particles at a time,
-//             counts them per (tile, class) key in an LDS hash table, reserves room with one global
-//             atomic per distinct key and writes the records (position, and weight).  Records beyond
+// 1. k_bin    particles -> bucket segments.  A workgroup takes BT * KP consecutive particles at a time,
+//             counts them per (tile, class) key in an LDS hash table (the positions of the next
+//             step are already loading), reserves room with one global atomic per distinct key and
+//             writes the records (position, and weight).  Records beyond
 //             a segment's capacity go to an overflow list.
 // 2. k_ov_*   the overflow list is grouped by tile with an exact count + scan (empty in the common
 //             case: the kernels then only read a zero counter).
@@ -24,8 +25,10 @@
 namespace pyl {
 
 constexpr int TILE = 16;
-constexpr int BT = 512;    // k_bin: threads per workgroup = particles per step
-constexpr int HT = 1024;   // k_bin: LDS hash slots (> BT, so an insertion always finds a slot)
+constexpr int BT = 512;    // k_bin: threads per workgroup
+constexpr int KP = 2;      // k_bin: particles per thread per step
+constexpr int HT = 2048;   // k_bin: LDS hash slots (> BT * KP, so an insertion always finds a slot)
+constexpr int TT = 512;    // k_tile: threads per workgroup
 
 // Lowest stencil cell (unwrapped) and the float64 1-D weights; the same arithmetic as the
 // stencils of deposit.hip (MAS_library.pyx:151-157, 290, 393-399, 486-492).
@@ -97,60 +100,93 @@ __device__ __forceinline__ int halo_class(int bx, int by, int bz) {
 template <int S, bool HASW>
 __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
     constexpr int R = HASW ? 4 : 3;
+    constexpr long long PS = (long long)BT * KP;   // particles per step
     __shared__ int hkey[HT];
     __shared__ unsigned hcnt[HT];
     __shared__ unsigned hbase[HT];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int i = tid; i < HT; i += BT) {
         hkey[i] = -1;
         hcnt[i] = 0u;
     }
     __syncthreads();
-    const long long nstep = (a.n + BT - 1) / BT;
-    for (long long step = blockIdx.x; step < nstep; step += gridDim.x) {
-        const long long p = step * BT + tid;
-        const bool valid = p < a.n;
-        float x = 0.f, y = 0.f, z = 0.f, w = 0.f;
-        int key = -1, slot = 0;
-        unsigned rank = 0u;
-        if (valid) {
-            x = a.pos[3 * p];
-            y = a.pos[3 * p + 1];
-            z = a.pos[3 * p + 2];
-            if (HASW) w = a.W[p];
-            double ww[4];
-            const int bx = wrapN(Axis<S>::eval((double)x * a.inv, ww), a.N);
-            const int by = wrapN(Axis<S>::eval((double)y * a.inv, ww), a.N);
-            const int bz = wrapN(Axis<S>::eval((double)z * a.inv, ww), a.N);
-            key = (((bx / TILE) * a.nt + by / TILE) * a.nt + bz / TILE) * 8 + halo_class<S>(bx, by, bz);
-            // at most BT distinct keys per step in HT > BT slots: the probe always ends
-            slot = (int)(((unsigned)key * 2654435761u) >> 22) & (HT - 1);
-            for (;;) {
-                const int prev = atomicCAS(&hkey[slot], -1, key);
-                if (prev == -1 || prev == key) break;
-                slot = (slot + 1) & (HT - 1);
+    const long long nstep = (a.n + PS - 1) / PS;
+    // particle k of this thread in a step: step * PS + k * BT + tid (coalesced for each k)
+    float px[KP], py[KP], pz[KP], pw[KP];
+    auto load = [&](long long step) {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const long long p = step * PS + k * BT + tid;
+            if (p < a.n) {
+                px[k] = a.pos[3 * p];
+                py[k] = a.pos[3 * p + 1];
+                pz[k] = a.pos[3 * p + 2];
+                pw[k] = HASW ? a.W[p] : 0.f;
             }
-            rank = atomicAdd(&hcnt[slot], 1u);
+        }
+    };
+    long long step = blockIdx.x;
+    if (step < nstep) load(step);
+    for (; step < nstep; step += gridDim.x) {
+        float x[KP], y[KP], z[KP], w[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            x[k] = px[k];
+            y[k] = py[k];
+            z[k] = pz[k];
+            w[k] = pw[k];
+        }
+        if (step + gridDim.x < nstep) load(step + gridDim.x);   // in flight during this step
+        int key[KP], slot[KP];
+        unsigned rank[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const long long p = step * PS + k * BT + tid;
+            const bool valid = p < a.n;
+            key[k] = -1;
+            slot[k] = 0;
+            rank[k] = 0u;
+            if (valid) {
+                double ww[4];
+                const int bx = wrapN(Axis<S>::eval((double)x[k] * a.inv, ww), a.N);
+                const int by = wrapN(Axis<S>::eval((double)y[k] * a.inv, ww), a.N);
+                const int bz = wrapN(Axis<S>::eval((double)z[k] * a.inv, ww), a.N);
+                key[k] = (((bx / TILE) * a.nt + by / TILE) * a.nt + bz / TILE) * 8 + halo_class<S>(bx, by, bz);
+            }
+            // at most BT * KP distinct keys per step in HT slots: the probe always ends
+            if (valid) {
+                int s0 = (int)(((unsigned)key[k] * 2654435761u) >> 21) & (HT - 1);
+                for (;;) {
+                    const int prev = atomicCAS(&hkey[s0], -1, key[k]);
+                    if (prev == -1 || prev == key[k]) break;
+                    s0 = (s0 + 1) & (HT - 1);
+                }
+                slot[k] = s0;
+                rank[k] = atomicAdd(&hcnt[s0], 1u);
+            }
         }
         __syncthreads();
         for (int i = tid; i < HT; i += BT) {
-            const int k = hkey[i];
-            if (k >= 0) hbase[i] = atomicAdd(&a.fill[k], hcnt[i]);
+            const int kk = hkey[i];
+            if (kk >= 0) hbase[i] = atomicAdd(&a.fill[kk], hcnt[i]);
         }
         __syncthreads();
-        if (valid) {
-            const int c = key & 7;
-            const unsigned idx = hbase[slot] + rank;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            if (key[k] < 0) continue;
+            const long long p = step * PS + k * BT + tid;
+            const int c = key[k] & 7;
+            const unsigned idx = hbase[slot[k]] + rank[k];
             if (idx < (unsigned)a.cap[c]) {
-                float* r = a.bucket + ((long long)(key >> 3) * a.CT + a.cofs[c] + idx) * R;
-                r[0] = x;
-                r[1] = y;
-                r[2] = z;
-                if (HASW) r[3] = w;
+                float* r = a.bucket + ((long long)(key[k] >> 3) * a.CT + a.cofs[c] + idx) * R;
+                r[0] = x[k];
+                r[1] = y[k];
+                r[2] = z[k];
+                if (HASW) r[3] = w[k];
             } else {
                 const unsigned o = atomicAdd(a.ov_count, 1u);
                 a.ov_idx[o] = p;
-                a.ov_key[o] = (unsigned)key;
+                a.ov_key[o] = (unsigned)key[k];
             }
         }
         __syncthreads();
@@ -246,7 +282,8 @@ struct TileArgs {
 };
 
 // the stencil weights of one particle that fall inside the tile with lowest cell (ox, oy, oz)
-template <int S, bool HASW>
+// (CLIP = false: a particle of the tile itself whose stencil stays inside it)
+template <int S, bool HASW, bool CLIP>
 __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, int ox, int oy, int oz, double inv,
                                            double mult) {
     double w0[S], w1[S], w2[S];
@@ -261,24 +298,24 @@ __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, i
 #pragma unroll
     for (int l = 0; l < S; ++l) {
         const int c0 = l0 + l;
-        if (c0 < 0 || c0 >= TILE) continue;
+        if (CLIP && (c0 < 0 || c0 >= TILE)) continue;
 #pragma unroll
         for (int m = 0; m < S; ++m) {
             const int c1 = l1 + m;
-            if (c1 < 0 || c1 >= TILE) continue;
+            if (CLIP && (c1 < 0 || c1 >= TILE)) continue;
             const double wlm = w0[l] * w1[m] * ww;
             double* row = acc + (c0 * TILE + c1) * TILE;
 #pragma unroll
             for (int q = 0; q < S; ++q) {
                 const int c2 = l2 + q;
-                if (c2 >= 0 && c2 < TILE) atomicAdd(row + c2, wlm * w2[q]);
+                if (!CLIP || (c2 >= 0 && c2 < TILE)) atomicAdd(row + c2, wlm * w2[q]);
             }
         }
     }
 }
 
 template <int S, typename G, bool HASW, int MODE>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a) {
+__global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     constexpr int R = HASW ? 4 : 3;
     constexpr int NE = 72;   // 8 neighbours (d = 0 own) x (8 class segments + 1 overflow group)
     __shared__ double acc[TILE * TILE * TILE];
@@ -288,7 +325,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     const int tid = threadIdx.x;
     const int t = blockIdx.x, nt = a.nt;
     const int tx = t / (nt * nt), ty = (t / nt) % nt, tz = t % nt;
-    for (int i = tid; i < TILE * TILE * TILE; i += 256) acc[i] = 0.0;
+    for (int i = tid; i < TILE * TILE * TILE; i += TT) acc[i] = 0.0;
     // entry e = d * 9 + c: class segment c (< 8) or the overflow group (c = 8) of the neighbour
     // t - d; a neighbour's particles reach this tile when their class contains d
     if (tid < NE) {
@@ -321,22 +358,56 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a) {
     __syncthreads();
     const unsigned tot = sstart[NE];
     const int ox = tx * TILE, oy = ty * TILE, oz = tz * TILE;
+    // two records per thread per step, both loaded before either is accumulated
     int e = 0;
-    for (unsigned f = tid; f < tot; f += 256) {
-        while (f >= sstart[e + 1]) ++e;
-        const long long i = sbase[e] + (f - sstart[e]);
-        if (e % 9 < 8) {
-            accumulate<S, HASW>(acc, a.bucket + i * R, a.N, ox, oy, oz, a.inv, a.mult);
-        } else {
-            const unsigned d = (unsigned)(e / 9);
-            if ((a.gcls[i] & d) == d) accumulate<S, HASW>(acc, a.grouped + i * R, a.N, ox, oy, oz, a.inv, a.mult);
+    for (unsigned f0 = tid; f0 < tot; f0 += 2 * TT) {
+        const unsigned f1 = f0 + TT;
+        const bool has1 = f1 < tot;
+        while (f0 >= sstart[e + 1]) ++e;
+        const int e0 = e;
+        int e1 = e;
+        if (has1)
+            while (f1 >= sstart[e1 + 1]) ++e1;
+        float r0[4], r1[4];
+        bool ok0 = true, ok1 = has1;
+        {
+            const long long i = sbase[e0] + (f0 - sstart[e0]);
+            const float* q = (e0 % 9 < 8) ? a.bucket + i * R : a.grouped + i * R;
+#pragma unroll
+            for (int c = 0; c < R; ++c) r0[c] = q[c];
+            if (e0 % 9 == 8) {
+                const unsigned d = (unsigned)(e0 / 9);
+                ok0 = (a.gcls[i] & d) == d;
+            }
         }
+        if (has1) {
+            const long long i = sbase[e1] + (f1 - sstart[e1]);
+            const float* q = (e1 % 9 < 8) ? a.bucket + i * R : a.grouped + i * R;
+#pragma unroll
+            for (int c = 0; c < R; ++c) r1[c] = q[c];
+            if (e1 % 9 == 8) {
+                const unsigned d = (unsigned)(e1 / 9);
+                ok1 = (a.gcls[i] & d) == d;
+            }
+        }
+        // entry 0: the tile's own class-0 particles, whose stencils stay inside the tile
+        if (e0 == 0)
+            accumulate<S, HASW, false>(acc, r0, a.N, ox, oy, oz, a.inv, a.mult);
+        else if (ok0)
+            accumulate<S, HASW, true>(acc, r0, a.N, ox, oy, oz, a.inv, a.mult);
+        if (has1) {
+            if (e1 == 0)
+                accumulate<S, HASW, false>(acc, r1, a.N, ox, oy, oz, a.inv, a.mult);
+            else if (ok1)
+                accumulate<S, HASW, true>(acc, r1, a.N, ox, oy, oz, a.inv, a.mult);
+        }
+        e = e1;
     }
     __syncthreads();
     G* grid = (G*)a.grid;
     const double am = (MODE == OUT_DELTA) ? (double)a.N * a.N * a.N / (a.dmass ? a.dmass[0] : a.mass) : 0.0;
     const long long N = a.N;
-    for (int idx = tid; idx < TILE * TILE * TILE; idx += 256) {
+    for (int idx = tid; idx < TILE * TILE * TILE; idx += TT) {
         const int i = idx / (TILE * TILE), j = (idx / TILE) % TILE, k = idx % TILE;
         const long long g = (((long long)ox + i) * N + oy + j) * N + oz + k;
         put<G, MODE>(grid, g, acc[idx], am);
@@ -390,8 +461,8 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
             ba.cofs[c] = cofs[c];
             ba.cap[c] = cap[c];
         }
-        long long blocks = (n + BT - 1) / BT;
-        if (blocks > 4096) blocks = 4096;
+        long long blocks = (n + (long long)BT * KP - 1) / ((long long)BT * KP);
+        if (blocks > 2048) blocks = 2048;
         hipLaunchKernelGGL((k_bin<S, HASW>), dim3((unsigned)blocks), dim3(BT), 0, st, ba);
         PYL_HIP(hipGetLastError());
     }
@@ -411,7 +482,7 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
             ta.cofs[c] = cofs[c];
             ta.cap[c] = cap[c];
         }
-        hipLaunchKernelGGL((k_tile<S, G, HASW, MODE>), dim3((unsigned)ntiles), dim3(256), 0, st, ta);
+        hipLaunchKernelGGL((k_tile<S, G, HASW, MODE>), dim3((unsigned)ntiles), dim3(TT), 0, st, ta);
         PYL_HIP(hipGetLastError());
     }
     return OK;
