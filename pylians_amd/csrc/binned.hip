@@ -34,33 +34,33 @@ constexpr int TT = 512;    // k_tile: threads per workgroup
 // stencils of deposit.hip (MAS_library.pyx:151-157, 290, 393-399, 486-492).
 template <int S> struct Axis;
 template <> struct Axis<1> {
-    static __device__ __forceinline__ long long eval(double d, double* w) {
+    static __device__ __forceinline__ double eval(double d, double* w) {
         w[0] = 1.0;
-        return (long long)floor(d + 0.5);
+        return floor(d + 0.5);
     }
 };
 template <> struct Axis<2> {
-    static __device__ __forceinline__ long long eval(double d, double* w) {
+    static __device__ __forceinline__ double eval(double d, double* w) {
         const double b = floor(d);
         const double u = d - b;
         w[0] = 1.0 - u;
         w[1] = u;
-        return (long long)b;
+        return b;
     }
 };
 template <> struct Axis<3> {
-    static __device__ __forceinline__ long long eval(double d, double* w) {
+    static __device__ __forceinline__ double eval(double d, double* w) {
         const double mn = floor(d - 1.5);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const double s = fabs(mn + (j + 1) - d);
             w[j] = s < 0.5 ? 0.75 - s * s : (s < 1.5 ? 0.5 * (1.5 - s) * (1.5 - s) : 0.0);
         }
-        return (long long)mn + 1;
+        return mn + 1.0;
     }
 };
 template <> struct Axis<4> {
-    static __device__ __forceinline__ long long eval(double d, double* w) {
+    static __device__ __forceinline__ double eval(double d, double* w) {
         const double mn = floor(d - 2.0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -68,13 +68,21 @@ template <> struct Axis<4> {
             w[j] = s < 1.0 ? (4.0 - 6.0 * s * s + 3.0 * s * s * s) / 6.0
                            : (s < 2.0 ? (2.0 - s) * (2.0 - s) * (2.0 - s) / 6.0 : 0.0);
         }
-        return (long long)mn + 1;
+        return mn + 1.0;
     }
 };
 
-__device__ __forceinline__ int wrapN(long long b, int N) {
-    long long r = b % N;
-    return (int)(r < 0 ? r + N : r);
+// periodic cell index of an integral float64 cell coordinate b (positions inside the box give
+// 0 <= b < N directly, a stencil edge at most one box away; others reduce with a float64 floor)
+__device__ __forceinline__ int wrapN(double b, int N) {
+    const double n = (double)N;
+    if (b >= 0.0 && b < n) return (int)b;
+    if (b < 0.0 && b >= -n) return (int)(b + n);
+    if (b >= n && b < 2.0 * n) return (int)(b - n);
+    int r = (int)(b - n * floor(b / n));
+    if (r >= N) r -= N;
+    if (r < 0) r += N;
+    return (unsigned)r < (unsigned)N ? r : 0;   // non-finite input: stay inside the grid
 }
 
 struct BinArgs {

@@ -12,7 +12,9 @@ namespace pyl {
 
 // ----------------------------------------------------------------------- kernels
 // Pass Z forward: NL rows of N = 2M reals, packed as M complex z[j] = x[2j] + i x[2j+1],
-// M-point FFT in LDS, then X[k] = E[k] + W_N^k O[k] for k in [0, M].
+// M-point FFT in LDS, then X[k] = E[k] + W_N^k O[k] for k in [0, M].  Persistent blocks walk row
+// blocks; the next block's rows are loaded (coalesced along the rows) into registers while the
+// current one is transformed.
 template <typename R, int M, int NL, int T>
 __global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R>* __restrict__ out,
                                                  long long nrows, long long P,
@@ -20,39 +22,66 @@ __global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R
                                                  const double* __restrict__ dmass, double mass, double ncell) {
     constexpr int N = 2 * M;
     constexpr int ES = NL + 1;
+    constexpr int V = (NL * M + T - 1) / T;   // complex inputs per thread per row block
     __shared__ cplx<R> s[M * ES];
     __shared__ cplx<R> tw[N];
     const int tid = threadIdx.x;
     // overdensity delta = rho/rhobar - 1, rhobar = mass/ncell (Pk_snapshot.py:84-88)
     const R a = affine ? (R)(ncell / (dmass ? dmass[0] : mass)) : (R)1;
     const R b = (R)-1;
-    const long long row0 = (long long)blockIdx.x * NL;
+    const long long nrb = (nrows + NL - 1) / NL;
     for (int i = tid; i < N; i += T) tw[i] = twg[i];
-    for (int idx = tid; idx < NL * M; idx += T) {
-        const int r = idx / M, j = idx % M;
-        const long long row = row0 + r;
-        cplx<R> v = mk<R>(0, 0);
-        if (row < nrows) {
-            const R* p = in + row * N + 2 * j;
-            v = mk(p[0], p[1]);
-            if (affine) {
-                v.x = v.x * a + b;
-                v.y = v.y * a + b;
+    cplx<R> nxt[V];
+    auto load = [&](long long rb) {
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int idx = tid + q * T;
+            const int r = idx / M, j = idx % M;
+            const long long row = rb * NL + r;
+            cplx<R> v = mk<R>(0, 0);
+            if ((NL * M) % T == 0 || idx < NL * M) {
+                if (row < nrows) {
+                    const R* p = in + row * N + 2 * j;
+                    v = mk(p[0], p[1]);
+                }
+            }
+            nxt[q] = v;
+        }
+    };
+    long long rb = blockIdx.x;
+    if (rb < nrb) load(rb);
+    for (; rb < nrb; rb += gridDim.x) {
+        cplx<R> cur[V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) cur[q] = nxt[q];
+        if (rb + gridDim.x < nrb) load(rb + gridDim.x);
+        __syncthreads();   // the previous row block's post-processing has read s
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int idx = tid + q * T;
+            if ((NL * M) % T == 0 || idx < NL * M) {
+                const int r = idx / M, j = idx % M;
+                cplx<R> v = cur[q];
+                if (affine) {
+                    v.x = v.x * a + b;
+                    v.y = v.y * a + b;
+                }
+                s[j * ES + r] = v;
             }
         }
-        s[j * ES + r] = v;
-    }
-    __syncthreads();
-    lds_fft<R, M, NL, ES, T, 2>(s, tw, tid);
-    for (int idx = tid; idx < NL * (M + 1); idx += T) {
-        const int r = idx / (M + 1), k = idx % (M + 1);
-        const long long row = row0 + r;
-        if (row >= nrows) continue;
-        const cplx<R> zk = s[(k % M) * ES + r];
-        const cplx<R> zm = conjg(s[((M - k) % M) * ES + r]);
-        const cplx<R> e = scale(zk + zm, (R)0.5);
-        const cplx<R> o = scale(mul_mi(zk - zm), (R)0.5);
-        out[row * P + k] = e + cmul(tw[k], o);
+        __syncthreads();
+        lds_fft<R, M, NL, ES, T, 2>(s, tw, tid);
+        const long long row0 = rb * NL;
+        for (int idx = tid; idx < NL * (M + 1); idx += T) {
+            const int r = idx / (M + 1), k = idx % (M + 1);
+            const long long row = row0 + r;
+            if (row >= nrows) continue;
+            const cplx<R> zk = s[(k % M) * ES + r];
+            const cplx<R> zm = conjg(s[((M - k) % M) * ES + r]);
+            const cplx<R> e = scale(zk + zm, (R)0.5);
+            const cplx<R> o = scale(mul_mi(zk - zm), (R)0.5);
+            out[row * P + k] = e + cmul(tw[k], o);
+        }
     }
 }
 
@@ -280,7 +309,14 @@ static int launch_r2c_rows(const R* in, cplx<R>* out, long long nrows, int64_t P
     constexpr int T = threads_for(M, NL);
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
-    dim3 grid((unsigned)((nrows + NL - 1) / NL));
+    // persistent grid: about as many blocks as fit on the chip at once (LDS bound)
+    constexpr int lds = (M * (NL + 1) + N) * (int)sizeof(cplx<R>);
+    int per_cu = (160 * 1024) / (lds + 1024);
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    long long nb = (long long)cu_count() * per_cu;
+    const long long nrb = (nrows + NL - 1) / NL;
+    if (nb > nrb) nb = nrb;
+    dim3 grid((unsigned)nb);
     hipLaunchKernelGGL((k_r2c_rows<R, M, NL, T>), grid, dim3(T), 0, st, in, out, nrows, (long long)P, tw,
                        affine ? 1 : 0, dmass, mass, (double)N * N * N);
     PYL_HIP(hipGetLastError());
