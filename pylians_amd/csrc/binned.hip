@@ -214,8 +214,11 @@ __global__ __launch_bounds__(256) void k_ov_count(const unsigned* __restrict__ o
         atomicAdd(&ocnt[ov_key[i] >> 3], 1u);
 }
 
-__global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ off, int n) {
+// (nothing overflowed: every group is empty and k_tile never reads an offset, so skip the scan)
+__global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ ov_count, const unsigned* __restrict__ cnt,
+                                               unsigned* __restrict__ off, int n) {
     __shared__ unsigned part[1024];
+    if (*ov_count == 0u) return;
     const int tid = threadIdx.x;
     const int per = (n + 1023) / 1024;
     const int lo = tid * per, hi = min(n, lo + per);
@@ -355,13 +358,20 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
         sbase[tid] = b;
     }
     __syncthreads();
-    if (tid == 0) {
-        unsigned run = 0u;
-        for (int e = 0; e < NE; ++e) {
-            sstart[e] = run;
-            run += scnt[e];
+    if (tid < 64) {
+        // exclusive prefix of the 72 counts: lane l holds entries 2l and 2l+1
+        const int e0 = 2 * tid, e1 = 2 * tid + 1;
+        const unsigned c0 = e0 < NE ? scnt[e0] : 0u, c1 = e1 < NE ? scnt[e1] : 0u;
+        unsigned v = c0 + c1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned u = __shfl_up(v, o, 64);
+            if (tid >= o) v += u;
         }
-        sstart[NE] = run;
+        const unsigned ex = v - c0 - c1;
+        if (e0 < NE) sstart[e0] = ex;
+        if (e1 < NE) sstart[e1] = ex + c0;
+        if (e1 == NE - 1) sstart[NE] = v;
     }
     __syncthreads();
     const unsigned tot = sstart[NE];
@@ -477,7 +487,7 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
     {
         StageTimer tm("deposit_overflow", st);
         hipLaunchKernelGGL(k_ov_count, dim3(256), dim3(256), 0, st, ov_count, ov_key, ocnt);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, ocnt, ooff, (int)ntiles);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, ov_count, ocnt, ooff, (int)ntiles);
         hipLaunchKernelGGL((k_ov_scatter<HASW>), dim3(256), dim3(256), 0, st, ov_count, ov_idx, ov_key, ooff, ocur,
                            pos, W, grouped, gcls);
         PYL_HIP(hipGetLastError());

@@ -340,9 +340,9 @@ static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P
 
 template <typename R, int N>
 static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st) {
-    constexpr int NL = Geo<R>::NL;
-    // the prefetch doubles the first stage's registers: keep it at 32 (f32) / 16 (f64) elements
-    constexpr int T = clamp_i(N * NL / (sizeof(R) == 4 ? 32 : 16), 64, 512);
+    constexpr int NL = (N >= 1024 && sizeof(R) == 4) ? 8 : Geo<R>::NL;   // 1024-point lines: 8 per block
+    // 16 elements per thread (the prefetch doubles the first stage's registers)
+    constexpr int T = clamp_i(N * NL / 16, 64, 512);
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
     const long long ncols = (long long)N * P;
