@@ -105,6 +105,15 @@ int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int
 int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
                      int prec, int flags, double* out, void* stream);
 
+/* Redshift space fused into the same path: pos_redshift_space (redshift_space_library.pyx:29-43)
+ * applied to each position as the deposit loads it -- x[rsd_axis] += vel[rsd_axis]*(1+z)/H in
+ * float32, then (x + L) mod L when it leaves [0, L] -- followed by pyl_pk_particles.  `pos` is not
+ * modified.  vel: (n, 3) float32 device array; Hubble in (km/s)/(Mpc/h).  The result equals
+ * pyl_pk_particles on the positions pos_redshift_space would produce. */
+int pyl_pk_particles_rsd(const float* pos, const float* vel, const float* W, int64_t n, int dims, double BoxSize,
+                         float Hubble, float redshift, int rsd_axis, int scheme, int axis, int prec, int flags,
+                         double* out, void* stream);
+
 /* Replaces Pk_library.XPk (Pk_library.pyx:534-798): auto and cross power spectra of nfields
  * (1..8) fields.  deltas: HOST array of nfields device pointers to (N,N,N) reals of precision prec;
  * mas_p: HOST array of window exponents.  out: HOST float64 array of pyl_xpk_layout values:

@@ -98,6 +98,7 @@ struct BinArgs {
     unsigned* ov_count;  // [1]
     long long* ov_idx;   // [n]
     unsigned* ov_key;    // [n]
+    Rsd rsd;             // redshift-space shift of the positions (rsd.vel == nullptr: none)
 };
 
 template <int S>
@@ -126,9 +127,14 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
         for (int k = 0; k < KP; ++k) {
             const long long p = step * PS + k * BT + tid;
             if (p < a.n) {
-                px[k] = a.pos[3 * p];
-                py[k] = a.pos[3 * p + 1];
-                pz[k] = a.pos[3 * p + 2];
+                float c[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
+                if (a.rsd.vel) {
+                    const int ax = a.rsd.axis;
+                    c[ax] = rsd_shift(c[ax], a.rsd.vel[3 * p + ax], a.rsd.fac, a.rsd.box);
+                }
+                px[k] = c[0];
+                py[k] = c[1];
+                pz[k] = c[2];
                 pw[k] = HASW ? a.W[p] : 0.f;
             }
         }
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(256) void k_ov_scatter(const unsigned* __restrict__
                                                      const unsigned* __restrict__ ov_key,
                                                      const unsigned* __restrict__ ooff, unsigned* __restrict__ ocur,
                                                      const float* __restrict__ pos, const float* __restrict__ W,
-                                                     float* __restrict__ grouped, unsigned* __restrict__ gcls) {
+                                                     float* __restrict__ grouped, unsigned* __restrict__ gcls, Rsd rsd) {
     constexpr int R = HASW ? 4 : 3;
     const long long n = *ov_count;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -255,9 +261,11 @@ __global__ __launch_bounds__(256) void k_ov_scatter(const unsigned* __restrict__
         const long long p = ov_idx[i];
         const unsigned s = ooff[t] + atomicAdd(&ocur[t], 1u);
         float* r = grouped + (long long)s * R;
-        r[0] = pos[3 * p];
-        r[1] = pos[3 * p + 1];
-        r[2] = pos[3 * p + 2];
+        float c[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        if (rsd.vel) c[rsd.axis] = rsd_shift(c[rsd.axis], rsd.vel[3 * p + rsd.axis], rsd.fac, rsd.box);
+        r[0] = c[0];
+        r[1] = c[1];
+        r[2] = c[2];
         if (HASW) r[3] = W[p];
         gcls[s] = key & 7u;
     }
@@ -437,7 +445,7 @@ bool binned_supported(int axes, int dims) { return axes == 3 && dims % TILE == 0
 
 template <int S, typename G, bool HASW, int MODE>
 static int run_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
-                      const double* dmass, double mass, hipStream_t st) {
+                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd) {
     constexpr int R = HASW ? 4 : 3;
     int rc = OK;
     const int nt = dims / TILE;
@@ -474,7 +482,7 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
     if (rc) return rc;
     {
         StageTimer tm("deposit_bin", st);
-        BinArgs ba{pos, W, (long long)n, dims, nt, dims / BoxSize, fill, bucket, CT, {}, {}, ov_count, ov_idx, ov_key};
+        BinArgs ba{pos, W, (long long)n, dims, nt, dims / BoxSize, fill, bucket, CT, {}, {}, ov_count, ov_idx, ov_key, rsd};
         for (int c = 0; c < 8; ++c) {
             ba.cofs[c] = cofs[c];
             ba.cap[c] = cap[c];
@@ -489,7 +497,7 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
         hipLaunchKernelGGL(k_ov_count, dim3(256), dim3(256), 0, st, ov_count, ov_key, ocnt);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, ov_count, ocnt, ooff, (int)ntiles);
         hipLaunchKernelGGL((k_ov_scatter<HASW>), dim3(256), dim3(256), 0, st, ov_count, ov_idx, ov_key, ooff, ocur,
-                           pos, W, grouped, gcls);
+                           pos, W, grouped, gcls, rsd);
         PYL_HIP(hipGetLastError());
     }
     {
@@ -508,37 +516,39 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
 
 template <int S, typename G, int MODE>
 static int dispatch_w(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
-                      const double* dmass, double mass, hipStream_t st) {
-    return W ? run_binned<S, G, true, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st)
-             : run_binned<S, G, false, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st);
+                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd) {
+    return W ? run_binned<S, G, true, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd)
+             : run_binned<S, G, false, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
 }
 
 template <typename G, int MODE>
 static int dispatch_s(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, double mult,
-                      void* grid, const double* dmass, double mass, hipStream_t st) {
+                      void* grid, const double* dmass, double mass, hipStream_t st, const Rsd& rsd) {
     switch (scheme) {
-        case 1: return dispatch_w<1, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st);
-        case 2: return dispatch_w<2, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st);
-        case 3: return dispatch_w<3, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st);
-        case 4: return dispatch_w<4, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st);
+        case 1: return dispatch_w<1, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
+        case 2: return dispatch_w<2, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
+        case 3: return dispatch_w<3, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
+        case 4: return dispatch_w<4, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
         default: set_error("unknown scheme %d", scheme); return E_ARG;
     }
 }
 
 // mode: 0 accumulate, 1 overwrite, 2 overdensity (mass = dmass[0] if dmass else mass)
 int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                   int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st) {
+                   int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st,
+                   const Rsd& rsd) {
+    PYL_CHECK(!rsd.vel || (rsd.axis >= 0 && rsd.axis <= 2), E_ARG, "redshift-space axis must be 0, 1 or 2");
     PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs 3-D grids with dims %% %d == 0", TILE);
     PYL_CHECK(n > 0, E_ARG, "binned deposit needs particles");
     PYL_CHECK(n < (1LL << 31), E_UNSUPPORTED, "binned deposit: at most 2^31 particles per call");
     if (f64) {
-        if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st);
-        if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st);
-        return dispatch_s<double, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st);
+        if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
+        if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
+        return dispatch_s<double, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
     }
-    if (mode == OUT_ACC) return dispatch_s<float, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st);
-    if (mode == OUT_SET) return dispatch_s<float, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st);
-    return dispatch_s<float, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st);
+    if (mode == OUT_ACC) return dispatch_s<float, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
+    if (mode == OUT_SET) return dispatch_s<float, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
+    return dispatch_s<float, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
 }
 
 }  // namespace pyl

@@ -227,20 +227,20 @@ bool g_tune_on = true;
 // (pass Z then applies delta = rho * a - 1 in float64).  The float32 atomic path accumulates in a
 // float64 grid and rounds delta once, like the binned path, so both give the same float32 field.
 static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                         bool binned, void* grid, const double* dmass, hipStream_t st) {
-    if (binned) return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st);
-    if (f64) return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, grid, st);
+                         bool binned, void* grid, const double* dmass, hipStream_t st, const Rsd& rsd = Rsd()) {
+    if (binned) return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st, rsd);
+    if (f64) return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, grid, st, rsd);
     const int64_t cells = (int64_t)dims * dims * dims;
     int rc = OK;
     double* rho = (double*)workspace(WS_AUX, sizeof(double) * (size_t)cells, &rc);
     if (rc) return rc;
-    rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, rho, st);
+    rc = deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, rho, st, rsd);
     if (rc) return rc;
     return delta_round(rho, (float*)grid, cells, dmass, (double)n, st);
 }
 
 static int tune_choice(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                       void* grid, const double* dmass, hipStream_t st, bool* binned) {
+                       void* grid, const double* dmass, hipStream_t st, bool* binned, const Rsd& rsd = Rsd()) {
     if (!binned_supported(3, dims)) {
         *binned = false;
         return OK;
@@ -273,11 +273,11 @@ static int tune_choice(const float* pos, const float* W, int64_t n, int dims, do
     float ms[2] = {0.f, 0.f};
     int rc = OK;
     for (int c = 0; c < 2 && rc == OK; ++c) {
-        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st);  // warm
+        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st, rsd);  // warm
         if (rc) break;
         PYL_HIP(hipEventRecord(e[0], st));
         for (int r = 0; r < 2 && rc == OK; ++r)
-            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st);
+            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st, rsd);
         PYL_HIP(hipEventRecord(e[1], st));
         PYL_HIP(hipEventSynchronize(e[1]));
         PYL_HIP(hipEventElapsedTime(&ms[c], e[0], e[1]));
@@ -323,14 +323,13 @@ extern "C" int pyl_tune_import(const char* text) {
     return OK;
 }
 
-extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
-                                int axis, int prec, int flags, double* out, void* stream) {
+static int pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
+                        int prec, int flags, double* out, hipStream_t st, const Rsd& rsd) {
     int rc = check_fft(dims, prec);
     if (rc) return rc;
     PYL_CHECK(axis >= 0 && axis <= 2, E_ARG, "axis must be 0, 1 or 2");
     PYL_CHECK(scheme >= 1 && scheme <= 4, E_ARG, "unknown mass-assignment scheme %d", scheme);
     PYL_CHECK(n > 0 && pos && out, E_ARG, "bad particle arguments");
-    hipStream_t st = (hipStream_t)stream;
     const size_t cells = (size_t)dims * dims * dims;
     const bool f64 = prec == PYL_F64;
     void* grid = workspace(WS_GRID, cells * (f64 ? 8 : 4), &rc);
@@ -351,10 +350,10 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
         PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
         binned = true;
     } else {
-        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &binned);
+        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &binned, rsd);
         if (rc) return rc;
     }
-    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, binned, grid, dmass, st);
+    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, binned, grid, dmass, st, rsd);
     if (rc) return rc;
     const int pkflags = flags & PYL_PK_ALL;
     if (f64)
@@ -362,6 +361,25 @@ extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int
                                      dmass, (double)n, st);
     return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, false,
                                 dmass, (double)n, st);
+}
+
+extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
+                                int axis, int prec, int flags, double* out, void* stream) {
+    return pk_particles(pos, W, n, dims, BoxSize, scheme, axis, prec, flags, out, (hipStream_t)stream, Rsd());
+}
+
+extern "C" int pyl_pk_particles_rsd(const float* pos, const float* vel, const float* W, int64_t n, int dims,
+                                    double BoxSize, float Hubble, float redshift, int rsd_axis, int scheme, int axis,
+                                    int prec, int flags, double* out, void* stream) {
+    PYL_CHECK(vel, E_ARG, "null velocity array");
+    PYL_CHECK(rsd_axis >= 0 && rsd_axis <= 2, E_ARG, "rsd_axis must be 0, 1 or 2");
+    PYL_CHECK(Hubble != 0.0f, E_ARG, "Hubble must be non-zero");
+    Rsd r;
+    r.vel = vel;
+    r.axis = rsd_axis;
+    r.fac = (float)((1.0 + (double)redshift) / (double)Hubble);   // redshift_space_library.pyx:36
+    r.box = (float)BoxSize;
+    return pk_particles(pos, W, n, dims, BoxSize, scheme, axis, prec, flags, out, (hipStream_t)stream, r);
 }
 
 // ------------------------------------------------------------------ adjoint and interpolation

@@ -68,13 +68,17 @@ template <> struct Stencil<4> {
 template <int S, typename G, bool HASW, int AXES>
 __global__ __launch_bounds__(256) void k_deposit_atomic(const float* __restrict__ pos, const float* __restrict__ W,
                                                          long long n, int dims, double inv, G mult,
-                                                         G* __restrict__ grid) {
+                                                         G* __restrict__ grid, Rsd rsd) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         int ix[3][S];
         G w[3][S];
 #pragma unroll
-        for (int a = 0; a < AXES; ++a) Stencil<S>::eval((double)pos[p * AXES + a] * inv, dims, ix[a], w[a]);
+        for (int a = 0; a < AXES; ++a) {
+            float x = pos[p * AXES + a];
+            if (rsd.vel && a == rsd.axis) x = rsd_shift(x, rsd.vel[p * 3 + a], rsd.fac, rsd.box);
+            Stencil<S>::eval((double)x * inv, dims, ix[a], w[a]);
+        }
         const G ww = HASW ? (G)W[p] * mult : mult;
         if (AXES == 3) {
 #pragma unroll
@@ -131,41 +135,44 @@ int delta_round(const double* rho, float* out, int64_t n, const double* dmass, d
 
 template <int S, typename G, bool HASW, int AXES>
 static int launch_atomic(const float* pos, const float* W, int64_t n, int dims, double inv, double mult, void* grid,
-                         hipStream_t st) {
+                         hipStream_t st, const Rsd& rsd) {
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL((k_deposit_atomic<S, G, HASW, AXES>), dim3((unsigned)blocks), dim3(256), 0, st, pos, W,
-                       (long long)n, dims, inv, (G)mult, (G*)grid);
+                       (long long)n, dims, inv, (G)mult, (G*)grid, rsd);
     PYL_HIP(hipGetLastError());
     return OK;
 }
 
 template <int S, typename G>
 static int dispatch_atomic(const float* pos, const float* W, int64_t n, int axes, int dims, double inv, double mult,
+                           const Rsd& rsd,
                            void* grid, hipStream_t st) {
     if (axes == 3)
-        return W ? launch_atomic<S, G, true, 3>(pos, W, n, dims, inv, mult, grid, st)
-                 : launch_atomic<S, G, false, 3>(pos, W, n, dims, inv, mult, grid, st);
-    return W ? launch_atomic<S, G, true, 2>(pos, W, n, dims, inv, mult, grid, st)
-             : launch_atomic<S, G, false, 2>(pos, W, n, dims, inv, mult, grid, st);
+        return W ? launch_atomic<S, G, true, 3>(pos, W, n, dims, inv, mult, grid, st, rsd)
+                 : launch_atomic<S, G, false, 3>(pos, W, n, dims, inv, mult, grid, st, rsd);
+    return W ? launch_atomic<S, G, true, 2>(pos, W, n, dims, inv, mult, grid, st, Rsd())
+             : launch_atomic<S, G, false, 2>(pos, W, n, dims, inv, mult, grid, st, Rsd());
 }
 
 template <typename G>
 static int dispatch_scheme(const float* pos, const float* W, int64_t n, int axes, int dims, double inv, double mult,
+                           const Rsd& rsd,
                            int scheme, void* grid, hipStream_t st) {
     switch (scheme) {
-        case 1: return dispatch_atomic<1, G>(pos, W, n, axes, dims, inv, mult, grid, st);
-        case 2: return dispatch_atomic<2, G>(pos, W, n, axes, dims, inv, mult, grid, st);
-        case 3: return dispatch_atomic<3, G>(pos, W, n, axes, dims, inv, mult, grid, st);
-        case 4: return dispatch_atomic<4, G>(pos, W, n, axes, dims, inv, mult, grid, st);
+        case 1: return dispatch_atomic<1, G>(pos, W, n, axes, dims, inv, mult, rsd, grid, st);
+        case 2: return dispatch_atomic<2, G>(pos, W, n, axes, dims, inv, mult, rsd, grid, st);
+        case 3: return dispatch_atomic<3, G>(pos, W, n, axes, dims, inv, mult, rsd, grid, st);
+        case 4: return dispatch_atomic<4, G>(pos, W, n, axes, dims, inv, mult, rsd, grid, st);
         default: set_error("unknown mass-assignment scheme %d", scheme); return E_ARG;
     }
 }
 
 int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, double BoxSize, int scheme, int flags,
-            void* grid, hipStream_t st) {
+            void* grid, hipStream_t st, const Rsd& rsd) {
     PYL_CHECK(axes == 2 || axes == 3, E_ARG, "axes must be 2 or 3 (got %d)", axes);
+    PYL_CHECK(!rsd.vel || (axes == 3 && rsd.axis >= 0 && rsd.axis <= 2), E_ARG, "redshift-space shift needs 3-D positions");
     PYL_CHECK(dims > 0, E_ARG, "dims must be positive");
     PYL_CHECK(BoxSize > 0, E_ARG, "BoxSize must be positive");
     PYL_CHECK(scheme >= 1 && scheme <= 4, E_ARG, "unknown mass-assignment scheme %d", scheme);
@@ -174,7 +181,7 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
     for (int a = 0; a < axes; ++a) cells *= dims;
     if (n > 0 && !(flags & DEP_ATOMIC) && binned_supported(axes, dims))
         return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, (flags & DEP_ZERO) ? 1 : 0, 1.0, grid, nullptr,
-                              0.0, st);
+                              0.0, st, rsd);
     PYL_CHECK(!(flags & DEP_BINNED), E_UNSUPPORTED, "binned deposit needs a 3-D grid with dims %% 16 == 0");
     if (flags & DEP_ZERO) {
         StageTimer t("grid_zero", st);
@@ -186,8 +193,8 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
     int rc = OK;
     if (n > 0) {
         StageTimer t("deposit_atomic", st);
-        rc = f64 ? dispatch_scheme<double>(pos, W, n, axes, dims, inv, mult, scheme, grid, st)
-                 : dispatch_scheme<float>(pos, W, n, axes, dims, inv, mult, scheme, grid, st);
+        rc = f64 ? dispatch_scheme<double>(pos, W, n, axes, dims, inv, mult, rsd, scheme, grid, st)
+                 : dispatch_scheme<float>(pos, W, n, axes, dims, inv, mult, rsd, scheme, grid, st);
         if (rc) return rc;
     }
     if (axes == 2 && (flags & DEP_DUP2D) && (flags & DEP_RENORM2D) && scheme > 1)

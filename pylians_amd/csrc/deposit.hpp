@@ -13,11 +13,29 @@ enum DepositFlags {
     DEP_BINNED = 32,    // force the cell-binned LDS kernel
 };
 
+// Redshift-space shift fused into the position load (redshift_space_library.pyx:29-43):
+// x[axis] <- x[axis] + vel[axis] * fac in float32, fac = (float)((1 + z) / H), and
+// (x + L) mod L when the result leaves [0, L].  vel == nullptr: no shift.
+struct Rsd {
+    const float* vel = nullptr;   // (n, 3) float32, row-major like the positions
+    int axis = 0;
+    float fac = 0.f;
+    float box = 0.f;
+};
+
+__device__ __forceinline__ float rsd_shift(float x, float v, float fac, float box) {
+#pragma clang fp contract(off)
+    const float dv = v * fac;   // two roundings (no fused multiply-add), as the reference's float arithmetic
+    float s = x + dv;
+    if (s > box || s < 0.0f) s = fmodf(s + box, box);
+    return s;
+}
+
 // pos: (n, axes) float32 row-major device array; W nullable; grid (dims^axes) float32 or float64.
 // scheme: 1 NGP, 2 CIC, 3 TSC, 4 PCS.  The cell index and sub-cell fractions are computed in
 // float64 from the float32 positions; weights are float32 (float64 grid: float64).
 int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, double BoxSize, int scheme,
-            int flags, void* grid, hipStream_t st);
+            int flags, void* grid, hipStream_t st, const Rsd& rsd = Rsd());
 
 // grid /= s (float32 or float64 grid of n cells), as numpy's `number2 /= 2.0` (MAS_library.pyx:89)
 int grid_divide(void* grid, int64_t n, bool f64, double s, hipStream_t st);
@@ -28,6 +46,7 @@ int delta_round(const double* rho, float* out, int64_t n, const double* dmass, d
 // 2 = grid = rho * N^3 / mass - 1 with mass = dmass[0] when dmass != nullptr, else `mass`.
 bool binned_supported(int axes, int dims);
 int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                   int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st);
+                   int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st,
+                   const Rsd& rsd = Rsd());
 
 }  // namespace pyl

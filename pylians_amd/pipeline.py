@@ -5,6 +5,9 @@ particles, forms delta = rho/rhobar - 1 with rhobar = sum(W)/N^3 (N_p/N^3 withou
 transforms and bins, all on the GPU in one stream, without materialising delta on the host.
 It exposes the same attributes as ``Pk_library.Pk``.
 
+With ``vel``, the redshift-space shift of pos_redshift_space (redshift_space_library.pyx:29-43)
+is applied to each position as the deposit loads it.
+
 ``run_device`` is the allocation-free inner call used by the benchmark: device inputs, a
 device output vector, no host synchronisation.
 """
@@ -17,25 +20,44 @@ from .Pk_library import pk_layout, unpack_pk
 PRECISION = {"fp32": _lib.F32, "fp64": _lib.F64}
 
 
-def run_device(pos, W, out, BoxSize, dims, scheme, axis, prec, flags=_lib.PK_ALL):
+def run_device(pos, W, out, BoxSize, dims, scheme, axis, prec, flags=_lib.PK_ALL, rsd=None):
     """pos: (n,3) float32 device tensor; W: (n,) float32 device tensor or None; out: float64 device
-    tensor of pk_layout(dims)[3] values."""
-    _lib.check(_lib.lib().pyl_pk_particles(_lib.ptr(pos), _lib.ptr(W), int(pos.shape[0]), int(dims),
-                                           float(BoxSize), int(scheme), int(axis), int(prec), int(flags),
-                                           _lib.ptr(out), _lib.stream()))
+    tensor of pk_layout(dims)[3] values; rsd: None or (vel (n,3) float32 device tensor, Hubble,
+    redshift, axis) for the fused redshift-space shift (redshift_space_library.pyx:29-43)."""
+    if rsd is None:
+        _lib.check(_lib.lib().pyl_pk_particles(_lib.ptr(pos), _lib.ptr(W), int(pos.shape[0]), int(dims),
+                                               float(BoxSize), int(scheme), int(axis), int(prec), int(flags),
+                                               _lib.ptr(out), _lib.stream()))
+        return
+    vel, hubble, redshift, rsd_axis = rsd
+    _lib.check(_lib.lib().pyl_pk_particles_rsd(_lib.ptr(pos), _lib.ptr(vel), _lib.ptr(W), int(pos.shape[0]),
+                                               int(dims), float(BoxSize), float(hubble), float(redshift),
+                                               int(rsd_axis), int(scheme), int(axis), int(prec), int(flags),
+                                               _lib.ptr(out), _lib.stream()))
 
 
 class PkParticles:
     def __init__(self, pos, BoxSize, dims, MAS="CIC", axis=0, W=None, precision="fp32", flags=_lib.PK_ALL,
-                 deposit=None):
+                 deposit=None, vel=None, Hubble=None, redshift=0.0, rsd_axis=None):
+        """With `vel`, the particles are moved to redshift space along `rsd_axis` (default: `axis`)
+        as pos_redshift_space(pos, vel, BoxSize, Hubble, redshift, rsd_axis) would
+        (redshift_space_library.pyx:29-43), fused into the deposit; `pos` is not modified."""
         if MAS not in _lib.SCHEMES:
             raise ValueError("unknown MAS %r" % (MAS,))
         p = _lib.to_dev(pos, torch.float32)
         if p.ndim != 2 or p.shape[1] != 3:
             raise ValueError("pos must be (n, 3)")
         w = None if W is None else _lib.to_dev(W, torch.float32)
+        rsd = None
+        if vel is not None:
+            if Hubble is None:
+                raise ValueError("redshift space needs Hubble")
+            v = _lib.to_dev(vel, torch.float32)
+            if tuple(v.shape) != tuple(p.shape):
+                raise ValueError("vel must have the shape of pos")
+            rsd = (v, Hubble, redshift, axis if rsd_axis is None else rsd_axis)
         _, _, _, nout = pk_layout(dims)
         out = _lib.empty((nout,), torch.float64)
         f = flags | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED}[deposit]
-        run_device(p, w, out, BoxSize, dims, _lib.SCHEMES[MAS], axis, PRECISION[precision], f)
+        run_device(p, w, out, BoxSize, dims, _lib.SCHEMES[MAS], axis, PRECISION[precision], f, rsd)
         unpack_pk(self, _lib.to_host(out), dims)

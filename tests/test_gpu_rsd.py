@@ -1,0 +1,48 @@
+"""Redshift space fused into the deposit: the same result as the path run on the positions that
+pos_redshift_space (redshift_space_library.pyx:29-43, oracle/rsd.py) produces -- the same float32
+positions and density grid; the float64 shell sums agree to their summation order (1e-12)."""
+import numpy as np
+import pytest
+
+from oracle import rsd
+from pylians_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n, L, seed):
+    pos = synth.uniform(n, L, seed=seed)
+    rng = np.random.default_rng(seed)
+    vel = (rng.standard_normal((n, 3)) * 300.0).astype(np.float32)
+    vel[:7, 0] = np.float32(2.0e5)      # a few particles that cross the box several times
+    vel[7:14, 0] = np.float32(-2.0e5)
+    return pos, vel
+
+
+@pytest.mark.parametrize("dims,MAS,prec", [(64, "CIC", "fp32"), (48, "TSC", "fp32"), (64, "PCS", "fp64"),
+                                           (32, "NGP", "fp32")])
+def test_fused_rsd_equals_shifted_positions(gpu, dims, MAS, prec):
+    from pylians_amd import pipeline
+    L, H, z = 250.0, 100.0, 0.5
+    pos, vel = _case(40000, L, dims)
+    ref_pos = rsd.pos_redshift_space(pos, vel, L, H, z, 0)
+    assert not np.array_equal(ref_pos, pos)
+    g = pipeline.PkParticles(pos, L, dims, MAS=MAS, axis=0, precision=prec, vel=vel, Hubble=H, redshift=z)
+    r = pipeline.PkParticles(ref_pos, L, dims, MAS=MAS, axis=0, precision=prec)
+    assert np.array_equal(g.Nmodes3D, r.Nmodes3D) and np.array_equal(g.k3D, r.k3D)
+    for name in ("Pk", "Pk1D", "Pkphase"):
+        assert np.allclose(getattr(g, name), getattr(r, name), rtol=1e-12, atol=0), name
+    # 2-D bin 0 holds only the DC mode: rounding noise of the zero mean
+    assert np.allclose(g.Pk2D[1:], r.Pk2D[1:], rtol=1e-12, atol=0)
+    assert abs(g.Pk2D[0] - r.Pk2D[0]) <= 1e-12 * np.abs(r.Pk2D).max()
+
+
+def test_fused_rsd_weighted_other_axis(gpu):
+    from pylians_amd import pipeline
+    L, H, z = 300.0, 70.0, 1.0
+    pos, vel = _case(30000, L, 5)
+    W = (np.random.default_rng(5).random(len(pos)) + 0.5).astype(np.float32)
+    ref_pos = rsd.pos_redshift_space(pos, vel, L, H, z, 2)
+    g = pipeline.PkParticles(pos, L, 64, MAS="CIC", axis=2, W=W, vel=vel, Hubble=H, redshift=z, rsd_axis=2)
+    r = pipeline.PkParticles(ref_pos, L, 64, MAS="CIC", axis=2, W=W)
+    assert np.allclose(g.Pk, r.Pk, rtol=1e-12, atol=0) and np.allclose(g.Pk2D[1:], r.Pk2D[1:], rtol=1e-12, atol=0)
