@@ -105,6 +105,25 @@ int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int
 int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
                      int prec, int flags, double* out, void* stream);
 
+/* Replaces Pk_library.correct_MAS (Pk_library.pyx:1749-1806): r2c of the (dims^3) field, every
+ * independent mode multiplied by the float window factor prod_i (x_i/sin x_i)^mas_p (the dependent
+ * half of the kz = 0 and N/2 planes unchanged, as the reference), 1/N^3-normalised c2r into `out`.
+ * delta, out: device arrays of precision prec (may not alias). */
+int pyl_correct_mas(const void* delta, void* out, int dims, int mas_p, int prec, void* stream);
+
+/* Replaces Pk_library.Xi (Pk_library.pyx:2035-2151).  out: HOST float64 array of 5*kmax values
+ * (pyl_xi_layout): r3D (kmax), xi (kmax x 3: monopole, quadrupole, hexadecapole, row-major),
+ * Nmodes3D (kmax); bins floor(|r|) in cell units, r = 0 bin dropped.  Synchronises `stream`. */
+int pyl_xi_layout(int dims, int* kmax, int64_t* out_len);
+int pyl_xi(const void* delta, int dims, double BoxSize, int mas_p, int axis, int prec, double* out, void* stream);
+
+/* Replace smoothing_library.FT_filter / field_smoothing (smoothing_library.pyx:20-114).
+ * pyl_ft_filter: top-hat (gaussian = 0) or Gaussian filter of radius R on the dims^3 grid,
+ * normalised by its float64 sum, r2c into out_k (complex64 (dims, dims, dims/2+1), device).
+ * pyl_field_smoothing: r2c(field) * filter_k, 1/N^3-normalised c2r into out (float32, device). */
+int pyl_ft_filter(int dims, float BoxSize, float R, int gaussian, void* out_k, void* stream);
+int pyl_field_smoothing(const void* field, const void* filter_k, void* out, int dims, void* stream);
+
 /* Redshift space fused into the same path: pos_redshift_space (redshift_space_library.pyx:29-43)
  * applied to each position as the deposit loads it -- x[rsd_axis] += vel[rsd_axis]*(1+z)/H in
  * float32, then (x + L) mod L when it leaves [0, L] -- followed by pyl_pk_particles.  `pos` is not

@@ -269,3 +269,42 @@ class Bk:
             raise ZeroDivisionError(_lib.lib().pyl_last_error().decode())
         _lib.check(rc)
         self.B, self.Q, self.k, self.Pk = B, Q, k, P
+
+
+def correct_MAS(delta, BoxSize, MAS="CIC", threads=1):
+    """The field with its mass-assignment window removed (Pk_library.pyx:1749-1806): the independent
+    modes of FFT(delta) times prod_i (x_i/sin x_i)^p, back to real space (1/N^3-normalised), on the
+    GPU.  float32 like the reference; float64 input stays float64."""
+    f64 = (delta.dtype == torch.float64) if isinstance(delta, torch.Tensor) else (delta.dtype == np.float64)
+    real_t = torch.float64 if f64 else torch.float32
+    d = _grid3(delta, real_t, "delta")
+    dims = int(d.shape[0])
+    out = _lib.empty((dims, dims, dims), real_t)
+    _lib.check(_lib.lib().pyl_correct_mas(_lib.ptr(d), _lib.ptr(out), dims, MAS_function(MAS),
+                                          _lib.F64 if f64 else _lib.F32, _lib.stream()))
+    return _ret(delta, out)
+
+
+class Xi:
+    """Correlation-function multipoles of a density field (Pk_library.pyx:2035-2151), on the GPU.
+
+    Attributes r3D (mean |r| of the cells in each bin, Mpc/h), xi (kmax, 3) monopole, quadrupole,
+    hexadecapole, Nmodes3D (cells per bin); bins floor(|r|/cell), the r = 0 bin dropped."""
+
+    def __init__(self, delta, BoxSize, MAS="CIC", axis=2, threads=1):
+        import ctypes
+        BoxSize = float(np.float32(BoxSize))   # a C float in the reference (:2036)
+        f64 = (delta.dtype == torch.float64) if isinstance(delta, torch.Tensor) else (delta.dtype == np.float64)
+        d = _grid3(delta, torch.float64 if f64 else torch.float32, "delta")
+        dims = int(d.shape[0])
+        kmax = ctypes.c_int()
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().pyl_xi_layout(dims, ctypes.byref(kmax), ctypes.byref(n)))
+        kmax = kmax.value
+        out = np.zeros(n.value, dtype=np.float64)
+        _lib.check(_lib.lib().pyl_xi(_lib.ptr(d), dims, BoxSize, MAS_function(MAS), int(axis),
+                                     _lib.F64 if f64 else _lib.F32, out.ctypes.data_as(ctypes.c_void_p),
+                                     _lib.stream()))
+        self.r3D = out[:kmax].copy()
+        self.xi = out[kmax:4 * kmax].reshape(kmax, 3).copy()
+        self.Nmodes3D = out[4 * kmax:].copy()

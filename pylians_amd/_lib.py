@@ -39,6 +39,11 @@ SIGNATURES = {
     "pyl_pk": [_vp, _i, _d, _i, _i, _i, _i, _vp, _vp, _vp],
     "pyl_pk_particles": [_vp, _vp, _i64, _i, _d, _i, _i, _i, _i, _vp, _vp],
     "pyl_pk_particles_rsd": [_vp, _vp, _vp, _i64, _i, _d, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
+    "pyl_correct_mas": [_vp, _vp, _i, _i, _i, _vp],
+    "pyl_xi_layout": [_i, _vp, _vp],
+    "pyl_xi": [_vp, _i, _d, _i, _i, _i, _vp, _vp],
+    "pyl_ft_filter": [_i, _f, _f, _i, _vp, _vp],
+    "pyl_field_smoothing": [_vp, _vp, _vp, _i, _vp],
     "pyl_bk": [_vp, _i, _d, _d, _d, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
     "pyl_pk_adjoint": [_vp, _i64, _i, _d, _i, _vp, _i, _vp, _vp, _vp],
     "pyl_interp": [_vp, _i, _d, _vp, _i64, _i, _vp, _vp],

@@ -109,8 +109,14 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
         const long long row = row0 + r;
         cplx<R> v = mk<R>(0, 0);
         if (row < nrows) {
-            const cplx<R> A = in[row * P + k];
-            const cplx<R> B = conjg(in[row * P + (M - k)]);
+            cplx<R> A = in[row * P + k];
+            cplx<R> B = conjg(in[row * P + (M - k)]);
+            // the imaginary parts of the DC and Nyquist terms of a row do not exist in a real
+            // transform: drop them, as numpy's irfftn and FFTW's c2r do for non-Hermitian input
+            if (k == 0) {
+                A.y = (R)0;
+                B.y = (R)0;
+            }
             const cplx<R> e = A + B;
             const cplx<R> o = cmul(A - B, conjg(tw[k]));
             v = conjg(e + mul_pi(o));
