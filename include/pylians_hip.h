@@ -124,6 +124,15 @@ int pyl_xi(const void* delta, int dims, double BoxSize, int mas_p, int axis, int
 int pyl_ft_filter(int dims, float BoxSize, float R, int gaussian, void* out_k, void* stream);
 int pyl_field_smoothing(const void* field, const void* filter_k, void* out, int dims, void* stream);
 
+/* Overdensity and field combinations of the snapshot drivers, with numpy's float32 rounding:
+ * pyl_field_div_sub: field = field / div - sub in place, div and sub rounded to the grid type and
+ * each operation rounded once -- `delta /= mean; delta -= 1.0` (Pk_snapshot.py:88, 194) and
+ * `delta_tot /= Omega_tot` (Pk_snapshot.py:256, sub = 0).
+ * pyl_field_axpy: y = y + a * x with the product rounded before the sum -- `delta_tot +=
+ * Omega * delta` (Pk_snapshot.py:252).  n cells of precision prec (PYL_F32 | PYL_F64), device. */
+int pyl_field_div_sub(void* field, int64_t n, double div, double sub, int prec, void* stream);
+int pyl_field_axpy(void* y, const void* x, int64_t n, double a, int prec, void* stream);
+
 /* Redshift space fused into the same path: pos_redshift_space (redshift_space_library.pyx:29-43)
  * applied to each position as the deposit loads it -- x[rsd_axis] += vel[rsd_axis]*(1+z)/H in
  * float32, then (x + L) mod L when it leaves [0, L] -- followed by pyl_pk_particles.  `pos` is not
@@ -132,6 +141,13 @@ int pyl_field_smoothing(const void* field, const void* filter_k, void* out, int 
 int pyl_pk_particles_rsd(const float* pos, const float* vel, const float* W, int64_t n, int dims, double BoxSize,
                          float Hubble, float redshift, int rsd_axis, int scheme, int axis, int prec, int flags,
                          double* out, void* stream);
+
+/* Replaces redshift_space_library.pos_redshift_space(pos, vel, BoxSize, Hubble, redshift, axis)
+ * (redshift_space_library.pyx:29-43): in place, pos[i, axis] += vel[i, axis] * (float)((1+z)/H) in
+ * float32 (product and sum rounded separately), then (pos + L) mod L (fmod) where the result
+ * leaves [0, L].  pos, vel: (n, cols) float32 row-major device arrays, 0 <= axis < cols <= 3. */
+int pyl_pos_redshift_space(float* pos, const float* vel, int64_t n, int cols, float BoxSize, float Hubble,
+                           float redshift, int axis, void* stream);
 
 /* Replaces Pk_library.XPk (Pk_library.pyx:534-798): auto and cross power spectra of nfields
  * (1..8) fields.  deltas: HOST array of nfields device pointers to (N,N,N) reals of precision prec;

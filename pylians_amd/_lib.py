@@ -39,6 +39,9 @@ SIGNATURES = {
     "pyl_pk": [_vp, _i, _d, _i, _i, _i, _i, _vp, _vp, _vp],
     "pyl_pk_particles": [_vp, _vp, _i64, _i, _d, _i, _i, _i, _i, _vp, _vp],
     "pyl_pk_particles_rsd": [_vp, _vp, _vp, _i64, _i, _d, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
+    "pyl_pos_redshift_space": [_vp, _vp, _i64, _i, _f, _f, _f, _i, _vp],
+    "pyl_field_div_sub": [_vp, _i64, _d, _d, _i, _vp],
+    "pyl_field_axpy": [_vp, _vp, _i64, _d, _i, _vp],
     "pyl_correct_mas": [_vp, _vp, _i, _i, _i, _vp],
     "pyl_xi_layout": [_i, _vp, _vp],
     "pyl_xi": [_vp, _i, _d, _i, _i, _i, _vp, _vp],

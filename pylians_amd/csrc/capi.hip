@@ -382,6 +382,16 @@ extern "C" int pyl_pk_particles_rsd(const float* pos, const float* vel, const fl
     return pk_particles(pos, W, n, dims, BoxSize, scheme, axis, prec, flags, out, (hipStream_t)stream, r);
 }
 
+extern "C" int pyl_pos_redshift_space(float* pos, const float* vel, int64_t n, int cols, float BoxSize, float Hubble,
+                                      float redshift, int axis, void* stream) {
+    PYL_CHECK(n >= 0, E_ARG, "negative particle count");
+    PYL_CHECK(n == 0 || (pos && vel), E_ARG, "null position or velocity array");
+    PYL_CHECK(cols >= 1 && cols <= 3 && axis >= 0 && axis < cols, E_ARG, "axis %d outside the %d columns", axis, cols);
+    PYL_CHECK(Hubble != 0.0f, E_ARG, "Hubble must be non-zero");
+    const float fac = (float)((1.0 + (double)redshift) / (double)Hubble);   // redshift_space_library.pyx:36
+    return pos_redshift_space(pos, vel, n, cols, axis, fac, BoxSize, (hipStream_t)stream);
+}
+
 // ------------------------------------------------------------------ adjoint and interpolation
 extern "C" int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxSize, int scheme, const double* w,
                               int nb, float* grad, double* out, void* stream) {

@@ -229,4 +229,26 @@ int weight_sum(const float* W, int64_t n, double* out, hipStream_t st) {
     return OK;
 }
 
+// pos_redshift_space in place (redshift_space_library.pyx:29-43): one thread per particle,
+// column `axis` of the (n, cols) float32 row-major arrays pos and vel.
+__global__ __launch_bounds__(256) void k_pos_rsd(float* __restrict__ pos, const float* __restrict__ vel, long long n,
+                                                 int cols, int axis, float fac, float box) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const long long j = i * cols + axis;
+        pos[j] = rsd_shift(pos[j], vel[j], fac, box);
+    }
+}
+
+int pos_redshift_space(float* pos, const float* vel, int64_t n, int cols, int axis, float fac, float box,
+                       hipStream_t st) {
+    if (n <= 0) return OK;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_pos_rsd, dim3((unsigned)blocks), dim3(256), 0, st, pos, vel, (long long)n, cols, axis, fac,
+                       box);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
 }  // namespace pyl

@@ -31,6 +31,10 @@ __device__ __forceinline__ float rsd_shift(float x, float v, float fac, float bo
     return s;
 }
 
+// pos[:, axis] shifted in place by rsd_shift with fac = (1 + z) / H; pos and vel (n, cols) float32.
+int pos_redshift_space(float* pos, const float* vel, int64_t n, int cols, int axis, float fac, float box,
+                       hipStream_t st);
+
 // pos: (n, axes) float32 row-major device array; W nullable; grid (dims^axes) float32 or float64.
 // scheme: 1 NGP, 2 CIC, 3 TSC, 4 PCS.  The cell index and sub-cell fractions are computed in
 // float64 from the float32 positions; weights are float32 (float64 grid: float64).

@@ -179,6 +179,27 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ f, long l
         f[i] = (float)((double)f[i] / s);
 }
 
+// numpy's in-place `x /= d; x -= s` on a grid (Pk_snapshot.py:88, 194, 256): d and s are already
+// rounded to the grid type, and each operation rounds once, as numpy's float32 ufuncs do.
+template <typename R>
+__global__ __launch_bounds__(256) void k_div_sub(R* __restrict__ f, long long n, R d, R s) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const R q = f[i] / d;
+        f[i] = q - s;
+    }
+}
+
+// numpy's `y += a * x` (Pk_snapshot.py:252): the product is rounded before the sum (no fused
+// multiply-add), a already rounded to the grid type.
+template <typename R>
+__global__ __launch_bounds__(256) void k_axpy(R* __restrict__ y, const R* __restrict__ x, long long n, R a) {
+#pragma clang fp contract(off)
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const R ax = a * x[i];
+        y[i] = y[i] + ax;
+    }
+}
+
 // ------------------------------------------------------------------ host side
 static unsigned grid_for(long long n) {
     long long b = (n + 255) / 256;
@@ -342,4 +363,34 @@ extern "C" int pyl_field_smoothing(const void* field, const void* filter_k, void
                        (const cplx<float>*)filter_k, dims, (long long)P);
     PYL_HIP(hipGetLastError());
     return fft3d_c2r<float>(spec, (float*)out, ws, dims, P, (float)(1.0 / ((double)dims * dims * dims)), st);
+}
+
+extern "C" int pyl_field_div_sub(void* field, int64_t n, double div, double sub, int prec, void* stream) {
+    PYL_CHECK(prec == PYL_F32 || prec == PYL_F64, E_ARG, "precision must be PYL_F32 or PYL_F64");
+    PYL_CHECK(n >= 0 && (n == 0 || field), E_ARG, "bad field");
+    if (n == 0) return OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == PYL_F32)
+        hipLaunchKernelGGL(k_div_sub<float>, dim3(grid_for(n)), dim3(256), 0, st, (float*)field, (long long)n,
+                           (float)div, (float)sub);
+    else
+        hipLaunchKernelGGL(k_div_sub<double>, dim3(grid_for(n)), dim3(256), 0, st, (double*)field, (long long)n, div,
+                           sub);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pyl_field_axpy(void* y, const void* x, int64_t n, double a, int prec, void* stream) {
+    PYL_CHECK(prec == PYL_F32 || prec == PYL_F64, E_ARG, "precision must be PYL_F32 or PYL_F64");
+    PYL_CHECK(n >= 0 && (n == 0 || (x && y)), E_ARG, "bad field");
+    if (n == 0) return OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == PYL_F32)
+        hipLaunchKernelGGL(k_axpy<float>, dim3(grid_for(n)), dim3(256), 0, st, (float*)y, (const float*)x,
+                           (long long)n, (float)a);
+    else
+        hipLaunchKernelGGL(k_axpy<double>, dim3(grid_for(n)), dim3(256), 0, st, (double*)y, (const double*)x,
+                           (long long)n, a);
+    PYL_HIP(hipGetLastError());
+    return OK;
 }

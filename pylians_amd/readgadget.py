@@ -1,6 +1,6 @@
 """Gadget snapshot reader (format 1 and 2 binary, either byte order, single or multi-file) with the
-call surface of the reference's readgadget (readgadget.py:17-154, readsnap.py): ``header(snapshot)``
-and ``read_block(snapshot, block, ptype)`` for the blocks the P(k) driver needs ("POS ", "VEL ",
+call surface of the reference's readgadget (readgadget.py:17-154, readsnap.py): ``header(snapshot)``,
+``read_field(file, block, ptype)`` and ``read_block(snapshot, block, ptype)`` for the blocks the P(k) driver needs ("POS ", "VEL ",
 "MASS", "ID  ").  Arrays are read with numpy memory maps, one contiguous slice per file and type.
 
 HDF5 snapshots need h5py, which this environment does not provide: they raise.
@@ -139,6 +139,21 @@ def _read_type(files, block, pt):
         del m
         parts.append(a.reshape(n_pt, 3) if vec else a)
     return np.concatenate(parts) if parts else np.zeros(0, np.float32)
+
+
+def read_field(snapshot, block, ptype):
+    """Block `block` of particle type `ptype` (an int) of ONE file of a snapshot (readgadget.py:63-97);
+    velocities times sqrt(a) when z != 0 (readsnap.py:375), masses from the mass table where it is set."""
+    if block not in _BLOCK_NUM:
+        raise Exception("block not implemented in readgadget!")
+    files, fformat = _files(snapshot)
+    if fformat == "hdf5":
+        raise Exception("HDF5 snapshots need h5py, which is not available")
+    out = _read_type(files[:1], block, int(ptype))
+    h = _Header(files[0])
+    if block == "VEL " and h.redshift != 0:
+        out *= np.float32(math.sqrt(h.time))
+    return out
 
 
 def read_block(snapshot, block, ptype, verbose=False):

@@ -46,3 +46,45 @@ def test_fused_rsd_weighted_other_axis(gpu):
     g = pipeline.PkParticles(pos, L, 64, MAS="CIC", axis=2, W=W, vel=vel, Hubble=H, redshift=z, rsd_axis=2)
     r = pipeline.PkParticles(ref_pos, L, 64, MAS="CIC", axis=2, W=W)
     assert np.allclose(g.Pk, r.Pk, rtol=1e-12, atol=0) and np.allclose(g.Pk2D[1:], r.Pk2D[1:], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("cols,axis", [(3, 0), (3, 2), (2, 1)])
+def test_pos_redshift_space_dropin(gpu, cols, axis):
+    """redshift_space_library.pos_redshift_space in place: bit-exact with the float32 oracle, for a
+    numpy array (written back) and a device tensor (shifted where it lies)."""
+    import torch
+    from pylians_amd import redshift_space_library as RSL
+    L, H, z = 500.0, 68.0, 2.0
+    pos, vel = _case(50000, L, 11 + cols)
+    pos, vel = np.ascontiguousarray(pos[:, :cols]), np.ascontiguousarray(vel[:, :cols])
+    pos[20, axis] = np.float32(L)        # x = L stays (only x > L wraps)
+    vel[20, axis] = np.float32(0.0)
+    want = rsd.pos_redshift_space(pos, vel, L, H, z, axis)
+    p = pos.copy()
+    assert RSL.pos_redshift_space(p, vel, L, H, z, axis) is None
+    assert np.array_equal(p, want)
+    t = torch.from_numpy(pos).cuda()
+    RSL.pos_redshift_space(t, torch.from_numpy(vel).cuda(), L, H, z, axis)
+    assert np.array_equal(t.cpu().numpy(), want)
+    with pytest.raises(ValueError):
+        RSL.pos_redshift_space(pos.astype(np.float64), vel, L, H, z, axis)
+
+
+def test_field_div_sub_axpy_match_numpy(gpu):
+    """The overdensity and combination steps of the snapshot drivers round like numpy float32."""
+    import torch
+    from pylians_amd import Pk_snapshot as PS
+    rng = np.random.default_rng(3)
+    a = (rng.random(100003) * 37.0).astype(np.float32)
+    b = rng.standard_normal(100003).astype(np.float32)
+    mean, om = 8.0 / 3.0 + 1e-3, 0.2718281828
+    t = torch.from_numpy(a).cuda()
+    PS._div_sub(t, mean, 1.0)
+    want = a.copy()
+    want /= np.float32(mean)
+    want -= np.float32(1.0)
+    assert np.array_equal(t.cpu().numpy(), want)
+    y = torch.from_numpy(want).cuda()
+    PS._axpy(y, torch.from_numpy(b).cuda(), om)
+    want2 = want + np.float32(om) * b
+    assert np.array_equal(y.cpu().numpy(), want2)

@@ -47,3 +47,20 @@ def test_reader_roundtrip(tmp_path, fmt, endian, nfiles):
 def test_missing_file_and_hdf5():
     with pytest.raises(Exception):
         readgadget.header("/nonexistent/snap")
+
+
+def test_read_field_per_file(tmp_path):
+    """read_field (readgadget.py:63-97) reads one file: the per-file pieces concatenate to read_block;
+    a base name resolves to its first file, as fname_format does."""
+    rng = np.random.default_rng(5)
+    n, pos, vel, ids, masses, massarr = _data(rng)
+    split = [np.array(n) // 2, np.array(n) - np.array(n) // 2]
+    base = str(tmp_path / "snap")
+    write_snapshot(base, pos, vel, ids, masses, massarr, split, 0.25, 3.0, 1000.0, fmt=2)
+    for t in (0, 1, 2, 4):
+        for block in ("POS ", "VEL ", "MASS", "ID  "):
+            parts = [readgadget.read_field("%s.%d" % (base, i), block, t) for i in range(2)]
+            assert np.array_equal(np.concatenate(parts), readgadget.read_block(base, block, [t])), (t, block)
+        assert np.array_equal(readgadget.read_field(base, "POS ", t), pos[t][:split[0][t]])
+    with pytest.raises(Exception):
+        readgadget.read_field(base, "RHO ", 0)
