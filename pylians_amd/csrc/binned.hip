@@ -29,6 +29,8 @@ constexpr int BT = 512;    // k_bin: threads per workgroup
 constexpr int KP = 2;      // k_bin: particles per thread per step
 constexpr int HT = 2048;   // k_bin: LDS hash slots (> BT * KP, so an insertion always finds a slot)
 constexpr int TT = 512;    // k_tile: threads per workgroup
+constexpr int XCDS = 8;    // XCDs of gfx950; workgroup i runs on XCD i % XCDS
+static_assert(HT % BT == 0, "hash slots per thread");
 
 // Lowest stencil cell (unwrapped) and the float64 1-D weights; the same arithmetic as the
 // stencils of deposit.hip (MAS_library.pyx:151-157, 290, 393-399, 486-492).
@@ -75,8 +77,12 @@ template <> struct Axis<4> {
 // periodic cell index of an integral float64 cell coordinate b (positions inside the box give
 // 0 <= b < N directly, a stencil edge at most one box away; others reduce with a float64 floor)
 __device__ __forceinline__ int wrapN(double b, int N) {
+    // v_cvt_i32_f64 clamps out-of-range values to INT_MIN / INT_MAX and turns NaN into 0, so one
+    // integer test takes every cell inside the grid (and NaN to cell 0, as below)
+    int i;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(i) : "v"(b));
+    if ((unsigned)i < (unsigned)N) return i;
     const double n = (double)N;
-    if (b >= 0.0 && b < n) return (int)b;
     if (b < 0.0 && b >= -n) return (int)(b + n);
     if (b >= n && b < 2.0 * n) return (int)(b - n);
     int r = (int)(b - n * floor(b / n));
@@ -113,7 +119,12 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
     __shared__ int hkey[HT];
     __shared__ unsigned hcnt[HT];
     __shared__ unsigned hbase[HT];
+    __shared__ int scap[8], scofs[8];
     const int tid = threadIdx.x, lane = tid & 63;
+    if (tid < 8) {
+        scap[tid] = a.cap[tid];
+        scofs[tid] = a.cofs[tid];
+    }
     for (int i = tid; i < HT; i += BT) {
         hkey[i] = -1;
         hcnt[i] = 0u;
@@ -139,9 +150,14 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
             }
         }
     };
-    long long step = blockIdx.x;
-    if (step < nstep) load(step);
-    for (; step < nstep; step += gridDim.x) {
+    // each XCD (workgroups go round-robin over the XCDS of the chip) takes one contiguous range of
+    // steps, so that records of one segment written at about the same time share an L2
+    const long long GX = gridDim.x / XCDS, per = (nstep + XCDS - 1) / XCDS;
+    const long long s_beg = (long long)(blockIdx.x % XCDS) * per;
+    const long long s_end = min(nstep, s_beg + per);
+    long long step = s_beg + blockIdx.x / XCDS;
+    if (step < s_end) load(step);
+    for (; step < s_end; step += GX) {
         float x[KP], y[KP], z[KP], w[KP];
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
@@ -150,7 +166,7 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
             z[k] = pz[k];
             w[k] = pw[k];
         }
-        if (step + gridDim.x < nstep) load(step + gridDim.x);   // in flight during this step
+        if (step + GX < s_end) load(step + GX);   // in flight during this step
         int key[KP], slot[KP];
         unsigned rank[KP];
 #pragma unroll
@@ -191,8 +207,8 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
             const long long p = step * PS + k * BT + tid;
             const int c = key[k] & 7;
             const unsigned idx = hbase[slot[k]] + rank[k];
-            if (idx < (unsigned)a.cap[c]) {
-                float* r = a.bucket + ((long long)(key[k] >> 3) * a.CT + a.cofs[c] + idx) * R;
+            if (idx < (unsigned)scap[c]) {
+                float* r = a.bucket + ((long long)(key[k] >> 3) * a.CT + scofs[c] + idx) * R;
                 r[0] = x[k];
                 r[1] = y[k];
                 r[2] = z[k];
@@ -302,7 +318,7 @@ struct TileArgs {
 
 // the stencil weights of one particle that fall inside the tile with lowest cell (ox, oy, oz)
 // (CLIP = false: a particle of the tile itself whose stencil stays inside it)
-template <int S, bool HASW, bool CLIP>
+template <int S, bool HASW, bool CLIP, bool UNIT>
 __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, int ox, int oy, int oz, double inv,
                                            double mult) {
     double w0[S], w1[S], w2[S];
@@ -322,7 +338,7 @@ __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, i
         for (int m = 0; m < S; ++m) {
             const int c1 = l1 + m;
             if (CLIP && (c1 < 0 || c1 >= TILE)) continue;
-            const double wlm = w0[l] * w1[m] * ww;
+            const double wlm = UNIT ? w0[l] * w1[m] : w0[l] * w1[m] * ww;   // UNIT: no weights, mult == 1
             double* row = acc + (c0 * TILE + c1) * TILE;
 #pragma unroll
             for (int q = 0; q < S; ++q) {
@@ -333,7 +349,7 @@ __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, i
     }
 }
 
-template <int S, typename G, bool HASW, int MODE>
+template <int S, typename G, bool HASW, int MODE, bool UNIT>
 __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     constexpr int R = HASW ? 4 : 3;
     constexpr int NE = 72;   // 8 neighbours (d = 0 own) x (8 class segments + 1 overflow group)
@@ -342,7 +358,11 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     __shared__ unsigned scnt[NE];
     __shared__ unsigned sstart[NE + 1];
     const int tid = threadIdx.x;
-    const int t = blockIdx.x, nt = a.nt;
+    // contiguous runs of tiles per XCD: the lower neighbours whose halo records a tile reads were
+    // mostly processed just before on the same XCD
+    const int nt = a.nt, ntl = nt * nt * nt, tper = (ntl + XCDS - 1) / XCDS;
+    const int t = (int)(blockIdx.x % XCDS) * tper + (int)(blockIdx.x / XCDS);
+    if (t >= ntl) return;
     const int tx = t / (nt * nt), ty = (t / nt) % nt, tz = t % nt;
     for (int i = tid; i < TILE * TILE * TILE; i += TT) acc[i] = 0.0;
     // entry e = d * 9 + c: class segment c (< 8) or the overflow group (c = 8) of the neighbour
@@ -384,50 +404,57 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     __syncthreads();
     const unsigned tot = sstart[NE];
     const int ox = tx * TILE, oy = ty * TILE, oz = tz * TILE;
-    // two records per thread per step, both loaded before either is accumulated
+    // RP records per thread per step; the records of the next step load while those of this step
+    // accumulate.  The loads are unconditional (an exhausted slot reads record 0 and class word 0,
+    // both always allocated) and their values are first used a step later, so only they stay in
+    // flight across the accumulation.
+    constexpr int RP = 2;
     int e = 0;
-    for (unsigned f0 = tid; f0 < tot; f0 += 2 * TT) {
-        const unsigned f1 = f0 + TT;
-        const bool has1 = f1 < tot;
-        while (f0 >= sstart[e + 1]) ++e;
-        const int e0 = e;
-        int e1 = e;
-        if (has1)
-            while (f1 >= sstart[e1 + 1]) ++e1;
-        float r0[4], r1[4];
-        bool ok0 = true, ok1 = has1;
-        {
-            const long long i = sbase[e0] + (f0 - sstart[e0]);
-            const float* q = (e0 % 9 < 8) ? a.bucket + i * R : a.grouped + i * R;
+    float cr[RP][4], nr[RP][4];
+    int ce[RP], ne[RP];
+    unsigned cg[RP], ng[RP];
+    auto fetch = [&](unsigned f) {
 #pragma unroll
-            for (int c = 0; c < R; ++c) r0[c] = q[c];
-            if (e0 % 9 == 8) {
-                const unsigned d = (unsigned)(e0 / 9);
-                ok0 = (a.gcls[i] & d) == d;
-            }
-        }
-        if (has1) {
-            const long long i = sbase[e1] + (f1 - sstart[e1]);
-            const float* q = (e1 % 9 < 8) ? a.bucket + i * R : a.grouped + i * R;
+        for (int j = 0; j < RP; ++j) {
+            const unsigned fj = f + j * TT;
+            const bool valid = fj < tot;
+            if (valid)
+                while (fj >= sstart[e + 1]) ++e;
+            const bool ovg = e % 9 == 8;
+            const long long i = valid ? sbase[e] + (fj - sstart[e]) : 0;
+            const float* q = (valid && ovg ? a.grouped : a.bucket) + i * R;
 #pragma unroll
-            for (int c = 0; c < R; ++c) r1[c] = q[c];
-            if (e1 % 9 == 8) {
-                const unsigned d = (unsigned)(e1 / 9);
-                ok1 = (a.gcls[i] & d) == d;
-            }
+            for (int c = 0; c < R; ++c) nr[j][c] = q[c];
+            ng[j] = a.gcls[valid && ovg ? i : 0];
+            ne[j] = valid ? e : -1;
         }
-        // entry 0: the tile's own class-0 particles, whose stencils stay inside the tile
-        if (e0 == 0)
-            accumulate<S, HASW, false>(acc, r0, a.N, ox, oy, oz, a.inv, a.mult);
-        else if (ok0)
-            accumulate<S, HASW, true>(acc, r0, a.N, ox, oy, oz, a.inv, a.mult);
-        if (has1) {
-            if (e1 == 0)
-                accumulate<S, HASW, false>(acc, r1, a.N, ox, oy, oz, a.inv, a.mult);
-            else if (ok1)
-                accumulate<S, HASW, true>(acc, r1, a.N, ox, oy, oz, a.inv, a.mult);
+    };
+    auto advance = [&]() {
+#pragma unroll
+        for (int j = 0; j < RP; ++j) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) cr[j][c] = nr[j][c];
+            ce[j] = ne[j];
+            cg[j] = ng[j];
         }
-        e = e1;
+    };
+    fetch(tid);
+    advance();
+    for (unsigned f = tid; f < tot; f += RP * TT) {
+        fetch(f + RP * TT);
+#pragma unroll
+        for (int j = 0; j < RP; ++j) {
+            const int ej = ce[j];
+            // entry 0: the tile's own class-0 particles, whose stencils stay inside the tile;
+            // an overflow group's record reaches this tile when its class contains the offset
+            const unsigned d = (unsigned)(ej / 9);
+            const bool ok = ej >= 0 && (ej % 9 < 8 || (cg[j] & d) == d);
+            if (ej == 0)
+                accumulate<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
+            else if (ok)
+                accumulate<S, HASW, true, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
+        }
+        advance();
     }
     __syncthreads();
     G* grid = (G*)a.grid;
@@ -489,6 +516,7 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
         }
         long long blocks = (n + (long long)BT * KP - 1) / ((long long)BT * KP);
         if (blocks > 2048) blocks = 2048;
+        blocks = (blocks + XCDS - 1) / XCDS * XCDS;
         hipLaunchKernelGGL((k_bin<S, HASW>), dim3((unsigned)blocks), dim3(BT), 0, st, ba);
         PYL_HIP(hipGetLastError());
     }
@@ -508,7 +536,11 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
             ta.cofs[c] = cofs[c];
             ta.cap[c] = cap[c];
         }
-        hipLaunchKernelGGL((k_tile<S, G, HASW, MODE>), dim3((unsigned)ntiles), dim3(TT), 0, st, ta);
+        const dim3 tg((unsigned)((ntiles + XCDS - 1) / XCDS * XCDS));
+        if (!HASW && mult == 1.0)
+            hipLaunchKernelGGL((k_tile<S, G, HASW, MODE, true>), tg, dim3(TT), 0, st, ta);
+        else
+            hipLaunchKernelGGL((k_tile<S, G, HASW, MODE, false>), tg, dim3(TT), 0, st, ta);
         PYL_HIP(hipGetLastError());
     }
     return OK;
