@@ -47,6 +47,8 @@ extern "C" {
 #define PYL_ZERO 8       /* zero the grid first (MA itself accumulates) */
 #define PYL_ATOMIC 16    /* force the direct global-atomic deposit */
 #define PYL_BINNED 32    /* force the cell-binned LDS deposit */
+#define PYL_SORTED 64    /* force the cell-binned LDS deposit after a presort by supertile (for
+                            particles in no spatial order) */
 
 /* Pk flags */
 #define PYL_PK_1D 1
@@ -181,7 +183,7 @@ typedef struct {
     int scheme;         /* 1..4, also the window exponent */
     int axis;
     int prec;           /* PYL_F32 / PYL_F64 */
-    int flags;          /* PYL_PK_* | PYL_ATOMIC / PYL_BINNED */
+    int flags;          /* PYL_PK_* | PYL_ATOMIC / PYL_BINNED / PYL_SORTED */
     double* out;        /* device, pyl_pk_layout(dims) values */
 } pyl_pk_job;
 
@@ -207,11 +209,12 @@ int pyl_interp(const float* density, int dims, double BoxSize, const float* pos,
                void* stream);
 
 /* ---------------------------------------------------------------- deposit autotuner
- * Builder-defined.  Without PYL_ATOMIC / PYL_BINNED, pyl_pk_particles picks the deposit kernel per
- * (dims, scheme, prec, weights, floor(log2 n)) by timing both once (outside graph capture) and
- * caching the faster.  pyl_tune(0) disables timing (binned is used where supported);
- * pyl_tune_export / pyl_tune_import move the cache as text lines "dims scheme prec hasw nlog choice"
- * (choice 0 = binned, 1 = atomic); export returns the full text length. */
+ * Builder-defined.  Without PYL_ATOMIC / PYL_BINNED / PYL_SORTED, pyl_pk_particles picks the
+ * deposit kernel per (dims, scheme, prec, weights, floor(log2 n)) by timing the three once (outside
+ * graph capture) and caching the fastest.  pyl_tune(0) disables timing (binned is used where
+ * supported); pyl_tune_export / pyl_tune_import move the cache as text lines
+ * "dims scheme prec hasw nlog choice" (choice 0 = binned, 1 = atomic, 2 = presorted binned);
+ * export returns the full text length. */
 int pyl_tune(int enable);
 int pyl_tune_export(char* buf, int len);
 int pyl_tune_import(const char* text);

@@ -18,6 +18,8 @@
 // overdensity delta = rho * a - 1 (a = N^3 / mass) evaluated in float64 before the single rounding.
 #include <math.h>
 
+#include <algorithm>
+
 #include "deposit.hpp"
 #include "profile.hpp"
 #include "util.hpp"
@@ -467,6 +469,128 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ presort for unordered inputs
+// Particles in no spatial order give k_bin one key per particle: one global atomic each and 12-byte
+// scattered records.  The presort first groups them by supertile (tps consecutive tiles, at most
+// SUPMAX groups) with an exact count, a scan and a scatter whose per-step chunks are contiguous; k_bin
+// then reads the grouped copy, where neighbouring particles share keys as in an ordered input.
+constexpr int SUPMAX = 1024;
+constexpr int PT = 512;   // presort kernels: threads per workgroup
+constexpr int PKP = 4;    // k_sup_scatter: particles per thread per step
+
+struct SupArgs {
+    const float* pos;
+    const float* W;
+    long long n;
+    int N, nt, tps, nsup;
+    double inv;
+    Rsd rsd;
+    unsigned* cnt;   // [nsup] particles per supertile
+    unsigned* off;   // [nsup] first slot of each supertile
+    unsigned* cur;   // [nsup] slots handed out so far
+    float* spos;     // [n * 3] grouped positions (redshift-space shift applied)
+    float* sw;       // [n] grouped weights (HASW)
+};
+
+__device__ __forceinline__ void sup_load(const SupArgs& a, long long p, float& x, float& y, float& z) {
+    float c[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
+    if (a.rsd.vel) c[a.rsd.axis] = rsd_shift(c[a.rsd.axis], a.rsd.vel[3 * p + a.rsd.axis], a.rsd.fac, a.rsd.box);
+    x = c[0];
+    y = c[1];
+    z = c[2];
+}
+
+// supertile of a particle: the tile k_bin gives it, divided by tps
+template <int S>
+__device__ __forceinline__ int sup_of(const SupArgs& a, float x, float y, float z) {
+    double ww[4];
+    const int bx = wrapN(Axis<S>::eval((double)x * a.inv, ww), a.N);
+    const int by = wrapN(Axis<S>::eval((double)y * a.inv, ww), a.N);
+    const int bz = wrapN(Axis<S>::eval((double)z * a.inv, ww), a.N);
+    return (((bx / TILE) * a.nt + by / TILE) * a.nt + bz / TILE) / a.tps;
+}
+
+template <int S>
+__global__ __launch_bounds__(PT) void k_sup_count(SupArgs a) {
+    __shared__ unsigned h[SUPMAX];
+    for (int i = threadIdx.x; i < a.nsup; i += PT) h[i] = 0u;
+    __syncthreads();
+    for (long long p = (long long)blockIdx.x * PT + threadIdx.x; p < a.n; p += (long long)gridDim.x * PT) {
+        float x, y, z;
+        sup_load(a, p, x, y, z);
+        atomicAdd(&h[sup_of<S>(a, x, y, z)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.nsup; i += PT)
+        if (h[i]) atomicAdd(&a.cnt[i], h[i]);
+}
+
+// exclusive scan of the nsup <= SUPMAX counts (one workgroup of SUPMAX threads)
+__global__ __launch_bounds__(SUPMAX) void k_sup_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ off,
+                                                     int nsup) {
+    __shared__ unsigned wsum[SUPMAX / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const unsigned c = tid < nsup ? cnt[tid] : 0u;
+    unsigned v = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    unsigned before = 0u;
+    for (int j = 0; j < wv; ++j) before += wsum[j];
+    if (tid < nsup) off[tid] = before + v - c;
+}
+
+template <int S, bool HASW>
+__global__ __launch_bounds__(PT) void k_sup_scatter(SupArgs a) {
+    constexpr long long PS = (long long)PT * PKP;
+    __shared__ unsigned lc[SUPMAX], lb[SUPMAX];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < a.nsup; i += PT) lc[i] = 0u;
+    __syncthreads();
+    const long long nstep = (a.n + PS - 1) / PS;
+    for (long long step = blockIdx.x; step < nstep; step += gridDim.x) {
+        float x[PKP], y[PKP], z[PKP], w[PKP];
+        int sp[PKP];
+        unsigned rank[PKP];
+#pragma unroll
+        for (int k = 0; k < PKP; ++k) {
+            const long long p = step * PS + k * PT + tid;
+            sp[k] = -1;
+            rank[k] = 0u;
+            if (p < a.n) {
+                sup_load(a, p, x[k], y[k], z[k]);
+                w[k] = HASW ? a.W[p] : 0.f;
+                sp[k] = sup_of<S>(a, x[k], y[k], z[k]);
+                rank[k] = atomicAdd(&lc[sp[k]], 1u);
+            }
+        }
+        __syncthreads();
+        // one reservation per supertile present in the step; the counters are cleared for the next
+        // step, whose insertions every thread makes only after its stores below
+        for (int i = tid; i < a.nsup; i += PT) {
+            const unsigned c = lc[i];
+            if (c) {
+                lb[i] = a.off[i] + atomicAdd(&a.cur[i], c);
+                lc[i] = 0u;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PKP; ++k) {
+            if (sp[k] < 0) continue;
+            const long long d = (long long)lb[sp[k]] + rank[k];
+            a.spos[3 * d] = x[k];
+            a.spos[3 * d + 1] = y[k];
+            a.spos[3 * d + 2] = z[k];
+            if (HASW) a.sw[d] = w[k];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ host side
 bool binned_supported(int axes, int dims) { return axes == 3 && dims % TILE == 0 && dims >= 2 * TILE; }
 
@@ -565,14 +689,70 @@ static int dispatch_s(const float* pos, const float* W, int64_t n, int dims, dou
     }
 }
 
+template <int S>
+static void launch_presort(const SupArgs& a, bool hasw, unsigned blocks, unsigned sblocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_sup_count<S>, dim3(blocks), dim3(PT), 0, st, a);
+    hipLaunchKernelGGL(k_sup_scan, dim3(1), dim3(SUPMAX), 0, st, a.cnt, a.off, a.nsup);
+    if (hasw)
+        hipLaunchKernelGGL((k_sup_scatter<S, true>), dim3(sblocks), dim3(PT), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sup_scatter<S, false>), dim3(sblocks), dim3(PT), 0, st, a);
+}
+
+// groups the particles by supertile into library workspaces (*spos, *sw); the redshift-space shift
+// is applied on the way, so the grouped copy is deposited without it
+static int presort(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, const Rsd& rsd,
+                   hipStream_t st, const float** spos, const float** sw) {
+    int rc = OK;
+    const int nt = dims / TILE;
+    const long long ntiles = (long long)nt * nt * nt;
+    // supertiles of whole tile columns (nt tiles along z), about 128 of them and never more than SUPMAX
+    long long tps = nt * std::max(1LL, (ntiles + 128LL * nt - 1) / (128LL * nt));
+    while ((ntiles + tps - 1) / tps > SUPMAX) tps += nt;
+    const int nsup = (int)((ntiles + tps - 1) / tps);
+    unsigned* cnt = (unsigned*)workspace(WS_SUP, sizeof(unsigned) * 3 * SUPMAX, &rc);
+    if (rc) return rc;
+    float* sp = (float*)workspace(WS_STAGE, sizeof(float) * (size_t)n * (W ? 4 : 3) + 64, &rc);
+    if (rc) return rc;
+    rc = zero_fill(cnt, sizeof(unsigned) * 3 * SUPMAX, st);
+    if (rc) return rc;
+    SupArgs a{pos, W, (long long)n, dims, nt, (int)tps, nsup, dims / BoxSize, rsd, cnt, cnt + SUPMAX,
+              cnt + 2 * SUPMAX, sp, W ? sp + 3 * n : nullptr};
+    long long blocks = (n + PT - 1) / PT;
+    if (blocks > 2048) blocks = 2048;
+    long long sblocks = (n + (long long)PT * PKP - 1) / ((long long)PT * PKP);
+    if (sblocks > 2048) sblocks = 2048;
+    StageTimer tm("deposit_presort", st);
+    switch (scheme) {
+        case 1: launch_presort<1>(a, W != nullptr, (unsigned)blocks, (unsigned)sblocks, st); break;
+        case 2: launch_presort<2>(a, W != nullptr, (unsigned)blocks, (unsigned)sblocks, st); break;
+        case 3: launch_presort<3>(a, W != nullptr, (unsigned)blocks, (unsigned)sblocks, st); break;
+        case 4: launch_presort<4>(a, W != nullptr, (unsigned)blocks, (unsigned)sblocks, st); break;
+        default: set_error("unknown scheme %d", scheme); return E_ARG;
+    }
+    PYL_HIP(hipGetLastError());
+    *spos = sp;
+    *sw = a.sw;
+    return OK;
+}
+
 // mode: 0 accumulate, 1 overwrite, 2 overdensity (mass = dmass[0] if dmass else mass)
 int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
                    int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st,
-                   const Rsd& rsd) {
-    PYL_CHECK(!rsd.vel || (rsd.axis >= 0 && rsd.axis <= 2), E_ARG, "redshift-space axis must be 0, 1 or 2");
+                   const Rsd& rsd_in, bool sorted) {
+    PYL_CHECK(!rsd_in.vel || (rsd_in.axis >= 0 && rsd_in.axis <= 2), E_ARG, "redshift-space axis must be 0, 1 or 2");
     PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs 3-D grids with dims %% %d == 0", TILE);
     PYL_CHECK(n > 0, E_ARG, "binned deposit needs particles");
     PYL_CHECK(n < (1LL << 31), E_UNSUPPORTED, "binned deposit: at most 2^31 particles per call");
+    Rsd rsd = rsd_in;
+    if (sorted) {
+        const float *sp = nullptr, *sw = nullptr;
+        int rc = presort(pos, W, n, dims, BoxSize, scheme, rsd_in, st, &sp, &sw);
+        if (rc) return rc;
+        pos = sp;
+        W = sw;
+        rsd = Rsd();
+    }
     if (f64) {
         if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
         if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);

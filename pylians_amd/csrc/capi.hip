@@ -19,7 +19,8 @@
 using namespace pyl;
 
 static_assert(PYL_GRID_F64 == DEP_GRID_F64 && PYL_DUP2D == DEP_DUP2D && PYL_RENORM2D == DEP_RENORM2D &&
-                  PYL_ZERO == DEP_ZERO && PYL_ATOMIC == DEP_ATOMIC && PYL_BINNED == DEP_BINNED,
+                  PYL_ZERO == DEP_ZERO && PYL_ATOMIC == DEP_ATOMIC && PYL_BINNED == DEP_BINNED &&
+                  PYL_SORTED == DEP_SORTED,
               "flag mismatch");
 static_assert(PYL_PK_1D == PK_1D && PYL_PK_2D == PK_2D && PYL_PK_PHASE == PK_PHASE, "flag mismatch");
 
@@ -208,9 +209,11 @@ extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int
 
 // ------------------------------------------------------------------ deposit autotuner
 // Per (device, dims, scheme, precision, weights, floor(log2 n)) the fused pipeline times the
-// cell-binned and the direct-atomic deposit once (both overwrite the grid, so the trial runs are
-// harmless), keeps the faster and caches the choice.  pyl_tune_export / pyl_tune_import move the
-// cache as text ("dims scheme prec hasw nlog choice" lines; choice 0 = binned, 1 = atomic).
+// cell-binned deposit, the direct-atomic deposit and the binned deposit after a presort by
+// supertile once (all overwrite the grid, so the trial runs are harmless), keeps the fastest and
+// caches the choice.  pyl_tune_export / pyl_tune_import move the cache as text ("dims scheme prec
+// hasw nlog choice" lines; choice 0 = binned, 1 = atomic, 2 = presorted binned).
+enum DepositKind { DK_BINNED = 0, DK_ATOMIC = 1, DK_SORTED = 2 };
 namespace {
 struct TuneKey {
     int dev, dims, scheme, prec, hasw, nlog;
@@ -227,8 +230,10 @@ bool g_tune_on = true;
 // (pass Z then applies delta = rho * a - 1 in float64).  The float32 atomic path accumulates in a
 // float64 grid and rounds delta once, like the binned path, so both give the same float32 field.
 static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                         bool binned, void* grid, const double* dmass, hipStream_t st, const Rsd& rsd = Rsd()) {
-    if (binned) return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st, rsd);
+                         int kind, void* grid, const double* dmass, hipStream_t st, const Rsd& rsd = Rsd()) {
+    if (kind != DK_ATOMIC)
+        return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st, rsd,
+                              kind == DK_SORTED);
     if (f64) return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, grid, st, rsd);
     const int64_t cells = (int64_t)dims * dims * dims;
     int rc = OK;
@@ -240,9 +245,9 @@ static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, 
 }
 
 static int tune_choice(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                       void* grid, const double* dmass, hipStream_t st, bool* binned, const Rsd& rsd = Rsd()) {
+                       void* grid, const double* dmass, hipStream_t st, int* kind, const Rsd& rsd = Rsd()) {
     if (!binned_supported(3, dims)) {
-        *binned = false;
+        *kind = DK_ATOMIC;
         return OK;
     }
     int dev = 0;
@@ -254,39 +259,42 @@ static int tune_choice(const float* pos, const float* W, int64_t n, int dims, do
         std::lock_guard<std::mutex> lk(g_tmu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) {
-            *binned = it->second == 0;
+            *kind = it->second;
             return OK;
         }
         if (!g_tune_on) {
-            *binned = true;
+            *kind = DK_BINNED;
             return OK;
         }
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     PYL_HIP(hipStreamIsCapturing(st, &cs));
     if (cs != hipStreamCaptureStatusNone) {   // never time inside a capture
-        *binned = true;
+        *kind = DK_BINNED;
         return OK;
     }
-    hipEvent_t e[3];
+    hipEvent_t e[2];
     for (auto& x : e) PYL_HIP(hipEventCreate(&x));
-    float ms[2] = {0.f, 0.f};
+    float ms[3] = {0.f, 0.f, 0.f};
     int rc = OK;
-    for (int c = 0; c < 2 && rc == OK; ++c) {
-        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st, rsd);  // warm
+    for (int c = 0; c < 3 && rc == OK; ++c) {
+        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd);  // warm
         if (rc) break;
         PYL_HIP(hipEventRecord(e[0], st));
         for (int r = 0; r < 2 && rc == OK; ++r)
-            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c == 0, grid, dmass, st, rsd);
+            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd);
         PYL_HIP(hipEventRecord(e[1], st));
         PYL_HIP(hipEventSynchronize(e[1]));
         PYL_HIP(hipEventElapsedTime(&ms[c], e[0], e[1]));
     }
     for (auto& x : e) (void)hipEventDestroy(x);
     if (rc) return rc;
-    *binned = ms[0] <= ms[1];
+    int best = DK_BINNED;
+    for (int c = 1; c < 3; ++c)
+        if (ms[c] < ms[best]) best = c;
+    *kind = best;
     std::lock_guard<std::mutex> lk(g_tmu);
-    g_tune[key] = *binned ? 0 : 1;
+    g_tune[key] = best;
     return OK;
 }
 
@@ -343,22 +351,22 @@ static int pk_particles(const float* pos, const float* W, int64_t n, int dims, d
     }
     // binned path: the deposit writes delta = rho/rhobar - 1 itself (float64 before one rounding);
     // atomic path: rho, normalised inside pass Z
-    bool binned;
+    int kind;
     if (flags & DEP_ATOMIC) {
-        binned = false;
-    } else if (flags & DEP_BINNED) {
+        kind = DK_ATOMIC;
+    } else if (flags & (DEP_BINNED | DEP_SORTED)) {
         PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
-        binned = true;
+        kind = (flags & DEP_SORTED) ? DK_SORTED : DK_BINNED;
     } else {
-        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &binned, rsd);
+        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &kind, rsd);
         if (rc) return rc;
     }
-    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, binned, grid, dmass, st, rsd);
+    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, kind, grid, dmass, st, rsd);
     if (rc) return rc;
     const int pkflags = flags & PYL_PK_ALL;
     if (f64)
-        return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, !binned,
-                                     dmass, (double)n, st);
+        return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr,
+                                     kind == DK_ATOMIC, dmass, (double)n, st);
     return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, false,
                                 dmass, (double)n, st);
 }
@@ -409,10 +417,10 @@ extern "C" int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxS
     cplx<float>* keep = (cplx<float>*)workspace(WS_SPEC2, sizeof(cplx<float>) * (size_t)dims * dims * P, &rc);
     if (rc) return rc;
     // forward: delta, spectrum (kept, window-corrected) and the binned outputs (out = P0 etc.)
-    const bool binned = binned_supported(3, dims);
+    const int kind = binned_supported(3, dims) ? DK_BINNED : DK_ATOMIC;
     {
         StageTimer t("adj_forward_deposit", st);
-        rc = deposit_delta(pos, nullptr, n, dims, BoxSize, scheme, false, binned, grid, nullptr, st);
+        rc = deposit_delta(pos, nullptr, n, dims, BoxSize, scheme, false, kind, grid, nullptr, st);
         if (rc) return rc;
     }
     rc = pk_from_delta<float>(grid, dims, BoxSize, 0, scheme, 0, out, keep, false, nullptr, (double)n, st);

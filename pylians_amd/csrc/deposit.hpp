@@ -11,6 +11,7 @@ enum DepositFlags {
     DEP_ZERO = 8,       // zero the grid before depositing (MA itself accumulates)
     DEP_ATOMIC = 16,    // force the direct global-atomic kernel
     DEP_BINNED = 32,    // force the cell-binned LDS kernel
+    DEP_SORTED = 64,    // force the cell-binned LDS kernel after a presort by supertile
 };
 
 // Redshift-space shift fused into the position load (redshift_space_library.pyx:29-43):
@@ -51,6 +52,6 @@ int delta_round(const double* rho, float* out, int64_t n, const double* dmass, d
 bool binned_supported(int axes, int dims);
 int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
                    int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st,
-                   const Rsd& rsd = Rsd());
+                   const Rsd& rsd = Rsd(), bool sorted = false);
 
 }  // namespace pyl

@@ -58,6 +58,6 @@ class PkParticles:
             rsd = (v, Hubble, redshift, axis if rsd_axis is None else rsd_axis)
         _, _, _, nout = pk_layout(dims)
         out = _lib.empty((nout,), torch.float64)
-        f = flags | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED}[deposit]
+        f = flags | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED, "presorted": _lib.SORTED}[deposit]
         run_device(p, w, out, BoxSize, dims, _lib.SCHEMES[MAS], axis, PRECISION[precision], f, rsd)
         unpack_pk(self, _lib.to_host(out), dims)

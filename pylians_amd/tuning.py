@@ -2,7 +2,7 @@
 
     from pylians_amd import tuning
     tuning.load()        # $PYLIANS_AMD_TUNE_CACHE or ~/.cache/pylians_amd/tune.json, if present
-    ...                  # first calls per workload shape time the binned and atomic deposits
+    ...                  # first calls per workload shape time the binned, atomic and presorted deposits
     tuning.save()
 """
 import ctypes
@@ -10,6 +10,9 @@ import json
 import os
 
 from . import _lib
+
+
+_KINDS = ["binned", "atomic", "presorted"]   # choice codes 0, 1, 2 of pyl_tune_export
 
 
 def default_path():
@@ -29,7 +32,7 @@ def entries():
     for line in export_text().splitlines():
         d, s, p, w, nl, c = (int(v) for v in line.split())
         out.append({"dims": d, "scheme": s, "prec": p, "weights": w, "log2_n": nl,
-                    "deposit": "binned" if c == 0 else "atomic"})
+                    "deposit": _KINDS[c]})
     return out
 
 
@@ -48,7 +51,7 @@ def load(path=None):
     with open(path) as f:
         ent = json.load(f)
     text = "".join("%d %d %d %d %d %d\n" % (e["dims"], e["scheme"], e["prec"], e["weights"], e["log2_n"],
-                                            0 if e["deposit"] == "binned" else 1) for e in ent)
+                                            _KINDS.index(e["deposit"])) for e in ent)
     _lib.check(_lib.lib().pyl_tune_import(text.encode()))
     return len(ent)
 

@@ -179,3 +179,41 @@ def test_binned_periodic_edges(gpu):
         o = C.deposit_f64(pos, dims, L, name)
         g = _dev_deposit(pos, None, dims, L, s, _lib.BINNED | _lib.ZERO)
         assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, name
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_presorted_binned_is_correctly_rounded(gpu, scheme, weighted):
+    """The binned deposit after the supertile presort (PYL_SORTED): same single rounding of the float64
+    cell sums, for unordered and ordered inputs, overwrite and accumulate, float32 and float64 grids."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(37)
+    n, dims, L = 300000, 80, 150.0
+    pos = rng.random((n, 3), dtype=np.float32) * np.float32(L) if weighted else \
+        synth.zeldovich(66, L, seed=9, rms_disp=3.0, sigma_v=100.0)[:n]
+    W = (rng.random(len(pos), dtype=np.float32) + np.float32(0.1)) if weighted else None
+    o = C.deposit_f64(pos, dims, L, scheme, W)
+    s = _lib.SCHEMES[scheme]
+    g = _dev_deposit(pos, W, dims, L, s, _lib.SORTED | _lib.ZERO)
+    assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8
+    start = rng.random((dims,) * 3).astype(np.float32)
+    gacc = _dev_deposit(pos, W, dims, L, s, _lib.SORTED, start=start)
+    assert np.max(np.abs(gacc - (start + o)) / (start + o)) <= 6.5e-8
+    g64 = _dev_deposit(pos, W, dims, L, s, _lib.SORTED | _lib.ZERO, f64=True)
+    assert np.allclose(g64, o, rtol=1e-13, atol=1e-13)
+
+
+def test_presorted_clustered_and_edges(gpu):
+    """Presort with overflowing buckets (half the particles in a few cells) and on the periodic edges
+    and the x = L face, against the float64 oracle."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(6)
+    n, dims, L = 400000, 64, 64.0
+    pos = np.concatenate([rng.random((n // 2, 3), dtype=np.float32) * np.float32(2.0) + np.float32(10.0),
+                          rng.random((n // 2, 3), dtype=np.float32) * np.float32(L)]).astype(np.float32)
+    edge = np.array([[0.0, 0.0, 0.0], [64.0, 64.0, 64.0], [63.99, 0.01, 63.5], [31.999, 32.0, 32.001]] * 50, np.float32)
+    for scheme, name in ((2, "CIC"), (4, "PCS")):
+        for p in (pos, edge):
+            o = C.deposit_f64(p, dims, L, name)
+            g = _dev_deposit(p, None, dims, L, scheme, _lib.SORTED | _lib.ZERO)
+            assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, name
