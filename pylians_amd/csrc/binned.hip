@@ -476,7 +476,7 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
 // then reads the grouped copy, where neighbouring particles share keys as in an ordered input.
 constexpr int SUPMAX = 1024;
 constexpr int PT = 512;   // presort kernels: threads per workgroup
-constexpr int PKP = 4;    // k_sup_scatter: particles per thread per step
+constexpr int PKP = 8;    // k_sup_scatter: particles per thread per step
 
 struct SupArgs {
     const float* pos;
@@ -500,13 +500,16 @@ __device__ __forceinline__ void sup_load(const SupArgs& a, long long p, float& x
     z = c[2];
 }
 
-// supertile of a particle: the tile k_bin gives it, divided by tps
+// supertile of a particle.  The grouping only has to be deterministic (the count and the scatter
+// must agree) and spatially coherent, not equal to k_bin's tiles, so float32 cells clamped into
+// the grid do (NaN goes to cell 0).
+__device__ __forceinline__ int sup_cell(float x, float inv, int N) {
+    return (int)fminf(fmaxf(x * inv, 0.0f), (float)(N - 1));
+}
 template <int S>
 __device__ __forceinline__ int sup_of(const SupArgs& a, float x, float y, float z) {
-    double ww[4];
-    const int bx = wrapN(Axis<S>::eval((double)x * a.inv, ww), a.N);
-    const int by = wrapN(Axis<S>::eval((double)y * a.inv, ww), a.N);
-    const int bz = wrapN(Axis<S>::eval((double)z * a.inv, ww), a.N);
+    const float inv = (float)a.inv;
+    const int bx = sup_cell(x, inv, a.N), by = sup_cell(y, inv, a.N), bz = sup_cell(z, inv, a.N);
     return (((bx / TILE) * a.nt + by / TILE) * a.nt + bz / TILE) / a.tps;
 }
 
