@@ -80,6 +80,17 @@ def cpu_leg(pos_host, N, L, axis, gpu_pk, threads):
                        "normalise + scipy.fft r2c (%d workers; FFTW absent) %.2fs + serial C restatement of "
                        "the Pk mode loop %.2fs" % (threads, t["deposit"], threads, t["normalise_fft"], t["modes"])),
         }
+        # the reference's scatter on one thread (its Cython MA loop is serial), on the first eighth
+        import time
+        sub = np.ascontiguousarray(pos_host[: len(pos_host) // 8])
+        grid = np.zeros((N, N, N), dtype=np.float32)
+        t0 = time.perf_counter()
+        C.ref_mas_c(sub, grid, L, "CIC", None, 1)
+        ts = time.perf_counter() - t0
+        out["cpu_serial_deposit"] = {"value": round(len(sub) / ts / 1e6, 3), "unit": "Mparticles/s", "cores": 1,
+                                     "kind": "reference",
+                                     "sample": "reference MAS_c.c CIC with threads=1 on the first %d particles "
+                                               "(deposit only)" % len(sub)}
     o, fl = C.fp64_oracle_pipeline(pos_host, N, L, axis, workers=threads, with_floor=True)
     P0 = o.Pk[:, 0]
     err = {
@@ -288,6 +299,19 @@ def main():
     unpack_pk(res64, out.cpu().numpy(), N)
     print("# fp64: %.3f ms/step" % ms64, file=sys.stderr, flush=True)
 
+    # ---- host drop-in (numpy positions in, Pk attributes out: PCIe both ways); reported beside the
+    # value, never as it
+    from pylians_amd.pipeline import PkParticles
+    pos_host = pos.cpu().numpy()
+    PkParticles(pos_host, L, N, MAS=args.mas, axis=axis)
+    torch.cuda.synchronize()
+    h0 = time.perf_counter()
+    for _ in range(3):
+        PkParticles(pos_host, L, N, MAS=args.mas, axis=axis)
+    torch.cuda.synchronize()
+    host_ms = (time.perf_counter() - h0) * 1e3 / 3
+    print("# host drop-in: %.2f ms/step" % host_ms, file=sys.stderr, flush=True)
+
     # ---- roofline of the dominant kernel
     ab = algorithmic_bytes(n_p, N, S=scheme)
     per = {k: (v[1] / v[0]) for k, v in stages.items() if v[0] > 0}
@@ -314,10 +338,11 @@ def main():
         "e2e_hbm_frac": round(ab["e2e"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "stage_ms": stage_ms, "roofline": roofline,
         "fp64_vs_fp32c_P0_maxrel": float(np.max(np.abs(res64.Pk[:, 0] - gpu_pk.Pk[:, 0]) / np.abs(res64.Pk[:, 0]))),
+        "host_dropin": {"ms_per_step": round(host_ms, 2), "Mparticles_s": round(n_p / (host_ms * 1e-3) / 1e6, 1),
+                        "note": "PkParticles on a pageable numpy array: positions host->device and results back"},
     }
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
-        pos_host = pos.cpu().numpy()
         t1 = time.time()
         line.update(cpu_leg(pos_host, N, L, axis, gpu_pk, threads))
         print("# cpu leg %.1fs" % (time.time() - t1), file=sys.stderr, flush=True)
