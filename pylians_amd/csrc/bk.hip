@@ -9,6 +9,8 @@
 // multiplies by the stored d1*d2 (resp. I1*I2) field instead of writing d3.
 #include <math.h>
 
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -24,7 +26,7 @@ const double* pk_window_table(int N, int p);
 
 template <typename R, int N, int NL, int T>
 __global__ __launch_bounds__(T) void k_shell_x_inv(const cplx<R>* __restrict__ spec, cplx<R>* __restrict__ dst,
-                                                    long long P, double kmin, double kmax, int indicator,
+                                                    long long P, int K, double kmin, double kmax, int indicator,
                                                     const double* __restrict__ wing,
                                                     const cplx<R>* __restrict__ twg) {
     __shared__ cplx<R> s[N * NL];
@@ -36,11 +38,13 @@ __global__ __launch_bounds__(T) void k_shell_x_inv(const cplx<R>* __restrict__ s
         win[i] = wing[i];
     }
     __syncthreads();
+    // columns (ky, kz < K); spectrum index c = ky * P + kz, x-stride N * P
     const long long ncols = (long long)N * P;
     const int l = tid % NL;
-    const long long c = (long long)blockIdx.x * NL + l;
-    const bool valid = c < ncols;
-    const int kyy = valid ? (int)(c / P) : 0, kzz = valid ? (int)(c % P) : 0;
+    const long long cc = (long long)blockIdx.x * NL + l;
+    const bool valid = cc < (long long)N * K;
+    const int kyy = valid ? (int)(cc / K) : 0, kzz = valid ? (int)(cc % K) : 0;
+    const long long c = (long long)kyy * P + kzz;
     const int ky = kfreq(kyy, N), kz = kzz;
     const double cyz = win[kyy] * win[kzz];
     for (int j = tid / NL; j < N; j += T / NL) {
@@ -64,15 +68,15 @@ __global__ __launch_bounds__(T) void k_shell_x_inv(const cplx<R>* __restrict__ s
 }
 
 template <typename R, int N>
-static int launch_shell(const cplx<R>* spec, cplx<R>* dst, int64_t P, double kmin, double kmax, int indicator,
+static int launch_shell(const cplx<R>* spec, cplx<R>* dst, int64_t P, int K, double kmin, double kmax, int indicator,
                         const double* win, hipStream_t st) {
     constexpr int NL = Geo<R>::NL;
     constexpr int T = threads_for(N, NL);
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
-    const long long ncols = (long long)N * P;
+    const long long ncols = (long long)N * K;
     hipLaunchKernelGGL((k_shell_x_inv<R, N, NL, T>), dim3((unsigned)((ncols + NL - 1) / NL)), dim3(T), 0, st, spec,
-                       dst, (long long)P, kmin, kmax, indicator, win, tw);
+                       dst, (long long)P, K, kmin, kmax, indicator, win, tw);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -82,12 +86,12 @@ static int launch_shell(const cplx<R>* spec, cplx<R>* dst, int64_t P, double kmi
     X(320) X(384) X(512) X(640) X(768) X(1024)
 
 template <typename R>
-static int shell_x_inv(const cplx<R>* spec, cplx<R>* dst, int N, int64_t P, double kmin, double kmax, int indicator,
-                       const double* win, hipStream_t st) {
+static int shell_x_inv(const cplx<R>* spec, cplx<R>* dst, int N, int64_t P, int K, double kmin, double kmax,
+                       int indicator, const double* win, hipStream_t st) {
     switch (N) {
 #define PYL_CASE(n) \
     case n:         \
-        return launch_shell<R, n>(spec, dst, P, kmin, kmax, indicator, win, st);
+        return launch_shell<R, n>(spec, dst, P, K, kmin, kmax, indicator, win, st);
         PYL_BK_SIZES(PYL_CASE)
 #undef PYL_CASE
         default:
@@ -97,16 +101,18 @@ static int shell_x_inv(const cplx<R>* spec, cplx<R>* dst, int N, int64_t P, doub
 }
 
 // One shell field through the inverse transform: first pass masked, then y, then rows with the
-// fused reductions / stores.
+// fused reductions / stores.  Every shell lies in kz < K: the first two passes transform only
+// those columns (all of them, every shell) and the rows pass reads only them.
 template <typename R>
-static int shell_field(const cplx<R>* spec, cplx<R>* ws, int N, int64_t P, double kmin, double kmax, int indicator,
-                       const double* win, R* out, int store, const R* aux, double* acc, hipStream_t st) {
-    int rc = shell_x_inv<R>(spec, ws, N, P, kmin, kmax, indicator, win, st);
+static int shell_field(const cplx<R>* spec, cplx<R>* ws, int N, int64_t P, int K, double kmin, double kmax,
+                       int indicator, const double* win, R* out, int store, const R* aux, double* acc,
+                       hipStream_t st) {
+    int rc = shell_x_inv<R>(spec, ws, N, P, K, kmin, kmax, indicator, win, st);
     if (rc) return rc;
-    rc = fft_pass_cols<R>(ws, ws, N, P, 1, true, (R)1, st);
+    rc = fft_pass_cols<R>(ws, ws, N, P, 1, true, (R)1, st, K);
     if (rc) return rc;
     const R sc = (R)(1.0 / ((double)N * N * N));
-    return fft_pass_z_c2r<R>(ws, out, N, P, sc, store, aux, acc, st);
+    return fft_pass_z_c2r<R>(ws, out, N, P, sc, store, aux, acc, st, K);
 }
 
 // The indicator fields I_i depend only on (N, L, k1, k2, theta): their sums (sum I_i^2 per shell,
@@ -148,6 +154,10 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
     if (rc) return rc;
     cplx<R>* ws = (cplx<R>*)workspace(WS_SPEC2, sizeof(cplx<R>) * nspec, &rc);
     if (rc) return rc;
+    // the shells reach |kz| < max kmax: columns kz >= K are zero in every masked spectrum
+    double kmx = 0.0;
+    for (int i = 0; i < nb + 2; ++i) kmx = std::max(kmx, kmax[i]);
+    const int K = (int)std::min<double>((double)(N / 2 + 1), std::floor(std::max(kmx, 0.0)) + 2.0);
     R* f = (R*)workspace(WS_AUX, sizeof(R) * ncell * 4, &rc);
     if (rc) return rc;
     R *d1 = f, *d12 = f + ncell, *I1 = f + 2 * ncell, *I12 = f + 3 * ncell;
@@ -170,16 +180,16 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
     rc = fft3d_r2c<R>(delta, spec, N, P, st);
     if (rc) return rc;
     // shell k1: d1 (and I1); shell k2: store d1*d2 (and I1*I2)
-    if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 0, win, d1, 1, nullptr, acc + 0, st))) return rc;
-    if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 0, win, d12, 2, d1, acc + 4, st))) return rc;
+    if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[0], kmax[0], 0, win, d1, 1, nullptr, acc + 0, st))) return rc;
+    if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[1], kmax[1], 0, win, d12, 2, d1, acc + 4, st))) return rc;
     if (need_i) {
-        if ((rc = shell_field<R>(spec, ws, N, P, kmin[0], kmax[0], 1, win, I1, 1, nullptr, acc + 2, st))) return rc;
-        if ((rc = shell_field<R>(spec, ws, N, P, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
+        if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[0], kmax[0], 1, win, I1, 1, nullptr, acc + 2, st))) return rc;
+        if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
     }
     for (int j = 0; j < nb; ++j) {
         double* a = acc + 4 * (j + 2);
-        if ((rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 0, win, nullptr, 0, d12, a, st))) return rc;
-        if (need_i && (rc = shell_field<R>(spec, ws, N, P, kmin[j + 2], kmax[j + 2], 1, win, nullptr, 0, I12, a + 2, st)))
+        if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[j + 2], kmax[j + 2], 0, win, nullptr, 0, d12, a, st))) return rc;
+        if (need_i && (rc = shell_field<R>(spec, ws, N, P, K, kmin[j + 2], kmax[j + 2], 1, win, nullptr, 0, I12, a + 2, st)))
             return rc;
     }
     std::vector<double> h(nacc);

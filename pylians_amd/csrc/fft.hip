@@ -7,6 +7,7 @@
 
 #include "fft.hpp"
 #include "fft_engine.hpp"
+#include "util.hpp"
 
 namespace pyl {
 
@@ -88,13 +89,13 @@ __global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R
 // Pass Z inverse: rows of M+1 complex -> N = 2M reals.
 // Z'[k] = (A + B) + i e^{+2 pi i k/N} (A - B), A = X[k], B = conj(X[M-k]); z = IFFT_M(Z');
 // x[2j] = Re z[j], x[2j+1] = Im z[j].  The inverse is computed as conj(FFT(conj(Z'))).
-// store: 0 = no output, 1 = x, 2 = x * aux.  Optional fused float64 reductions:
-// acc[0] += sum x^2, acc[1] += sum x*aux.
+// store: 0 = no output, 1 = x, 2 = x * aux.  Optional fused float64 reductions: each workgroup
+// writes its sum x^2 and sum x*aux to part[2 * block + 0 / 1] (k_c2r_partsum adds them up).
 template <typename R, int M, int NL, int T>
 __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, R* __restrict__ out,
                                                  long long nrows, long long P,
                                                  const cplx<R>* __restrict__ twg, R sc, int store,
-                                                 const R* __restrict__ aux, double* __restrict__ acc) {
+                                                 const R* __restrict__ aux, double* __restrict__ part, int K) {
     constexpr int N = 2 * M;
     constexpr int ES = NL + 1;
     __shared__ cplx<R> s[M * ES];
@@ -109,8 +110,9 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
         const long long row = row0 + r;
         cplx<R> v = mk<R>(0, 0);
         if (row < nrows) {
-            cplx<R> A = in[row * P + k];
-            cplx<R> B = conjg(in[row * P + (M - k)]);
+            // entries kz >= K are known zeros (K = M + 1: the whole row) and are not read
+            cplx<R> A = k < K ? in[row * P + k] : mk<R>(0, 0);
+            cplx<R> B = M - k < K ? conjg(in[row * P + (M - k)]) : mk<R>(0, 0);
             // the imaginary parts of the DC and Nyquist terms of a row do not exist in a real
             // transform: drop them, as numpy's irfftn and FFTW's c2r do for non-Hermitian input
             if (k == 0) {
@@ -143,12 +145,12 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
             p[0] = store == 2 ? x0 * q0 : x0;
             p[1] = store == 2 ? x1 * q1 : x1;
         }
-        if (acc) {
+        if (part) {
             a0 += (double)x0 * x0 + (double)x1 * x1;
             a1 += (double)x0 * q0 + (double)x1 * q1;
         }
     }
-    if (acc) {
+    if (part) {
         for (int off = 32; off > 0; off >>= 1) {
             a0 += __shfl_down(a0, off, 64);
             a1 += __shfl_down(a1, off, 64);
@@ -164,9 +166,40 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
                 s0 += red[0][w];
                 s1 += red[1][w];
             }
-            atomicAdd(acc, s0);
-            if (aux) atomicAdd(acc + 1, s1);
+            part[2 * blockIdx.x] = s0;
+            part[2 * blockIdx.x + 1] = s1;
         }
+    }
+}
+
+// acc[0] += sum of part[2 b], acc[1] += sum of part[2 b + 1] (when two) over the nb workgroups of
+// k_c2r_rows; one workgroup, one atomic per sum
+__global__ __launch_bounds__(1024) void k_c2r_partsum(const double* __restrict__ part, int nb, double* __restrict__ acc,
+                                                      int two) {
+    __shared__ double red[2][16];
+    const int tid = threadIdx.x;
+    double a0 = 0.0, a1 = 0.0;
+    for (int b = tid; b < nb; b += 1024) {
+        a0 += part[2 * b];
+        a1 += part[2 * b + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off, 64);
+        a1 += __shfl_down(a1, off, 64);
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = a0;
+        red[1][tid >> 6] = a1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < 16; ++w) {
+            s0 += red[0][w];
+            s1 += red[1][w];
+        }
+        atomicAdd(acc, s0);
+        if (two) atomicAdd(acc + 1, s1);
     }
 }
 
@@ -331,30 +364,43 @@ static int launch_r2c_rows(const R* in, cplx<R>* out, long long nrows, int64_t P
 
 template <typename R, int N>
 static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P, R sc, int store,
-                           const R* aux, double* acc, hipStream_t st) {
+                           const R* aux, double* acc, hipStream_t st, int K) {
     constexpr int M = N / 2;
     constexpr int NL = Geo<R>::NL;
     constexpr int T = threads_for(M, NL);
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
-    dim3 grid((unsigned)((nrows + NL - 1) / NL));
-    hipLaunchKernelGGL((k_c2r_rows<R, M, NL, T>), grid, dim3(T), 0, st, in, out, nrows, (long long)P, tw, sc,
-                       store, aux, acc);
+    const long long nb = (nrows + NL - 1) / NL;
+    double* part = nullptr;
+    if (acc) {
+        int rc = OK;
+        part = (double*)workspace(WS_C2R, sizeof(double) * 2 * (size_t)nb, &rc);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL((k_c2r_rows<R, M, NL, T>), dim3((unsigned)nb), dim3(T), 0, st, in, out, nrows, (long long)P, tw,
+                       sc, store, aux, part, (K > 0 && K <= M) ? K : M + 1);
+    if (acc)
+        hipLaunchKernelGGL(k_c2r_partsum, dim3(1), dim3(1024), 0, st, part, (int)nb, acc, aux ? 1 : 0);
     PYL_HIP(hipGetLastError());
     return OK;
 }
 
 template <typename R, int N>
-static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st) {
+static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st,
+                       int K) {
     constexpr int NL = (N >= 1024 && sizeof(R) == 4) ? 8 : Geo<R>::NL;   // 1024-point lines: 8 per block
     // 16 elements per thread (the prefetch doubles the first stage's registers)
     constexpr int T = clamp_i(N * NL / 16, 64, 512);
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
-    const long long ncols = (long long)N * P;
+    // K < W (pass along y only): just the columns kz < K, enumerated without the others
+    const bool lim = axis == 1 && K > 0 && K < N / 2 + 1;
+    const long long PW = lim ? K : P;
+    const int WV = lim ? K : N / 2 + 1;
+    const long long ncols = (long long)N * PW;
     long long CW, CS, EST;
     if (axis == 1) {
-        CW = P;
+        CW = PW;
         CS = (long long)N * P;
         EST = P;
     } else {
@@ -371,11 +417,11 @@ static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bo
     if (nb > nblk) nb = nblk;
     dim3 grid((unsigned)nb);
     if (inv)
-        hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST,
-                           (long long)P, N / 2 + 1, tw, sc);
+        hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, PW, WV,
+                           tw, sc);
     else
-        hipLaunchKernelGGL((k_cols<R, N, NL, T, false>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST,
-                           (long long)P, N / 2 + 1, tw, sc);
+        hipLaunchKernelGGL((k_cols<R, N, NL, T, false>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, PW, WV,
+                           tw, sc);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -398,11 +444,11 @@ int fft_pass_z_r2c(const R* in, cplx<R>* out, int N, int64_t P, bool affine, con
 
 template <typename R>
 int fft_pass_cols(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, bool inverse, R sc,
-                  hipStream_t st) {
+                  hipStream_t st, int K) {
     switch (N) {
 #define PYL_CASE(n) \
     case n:         \
-        return launch_cols<R, n>(src, dst, P, axis, inverse, sc, st);
+        return launch_cols<R, n>(src, dst, P, axis, inverse, sc, st, K);
         PYL_FFT_SIZES(PYL_CASE)
 #undef PYL_CASE
         default:
@@ -413,12 +459,12 @@ int fft_pass_cols(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, 
 
 template <typename R>
 int fft_pass_z_c2r(const cplx<R>* in, R* out, int N, int64_t P, R sc, int store, const R* aux, double* acc,
-                   hipStream_t st) {
+                   hipStream_t st, int K) {
     const long long nrows = (long long)N * N;
     switch (N) {
 #define PYL_CASE(n) \
     case n:         \
-        return launch_c2r_rows<R, n>(in, out, nrows, P, sc, store, aux, acc, st);
+        return launch_c2r_rows<R, n>(in, out, nrows, P, sc, store, aux, acc, st, K);
         PYL_FFT_SIZES(PYL_CASE)
 #undef PYL_CASE
         default:
@@ -447,8 +493,8 @@ int fft3d_c2r(const cplx<R>* in, R* out, cplx<R>* ws, int N, int64_t P, R sc, hi
 
 #define PYL_INST(R)                                                                                        \
     template int fft_pass_z_r2c<R>(const R*, cplx<R>*, int, int64_t, bool, const double*, double, hipStream_t);               \
-    template int fft_pass_cols<R>(const cplx<R>*, cplx<R>*, int, int64_t, int, bool, R, hipStream_t);         \
-    template int fft_pass_z_c2r<R>(const cplx<R>*, R*, int, int64_t, R, int, const R*, double*, hipStream_t); \
+    template int fft_pass_cols<R>(const cplx<R>*, cplx<R>*, int, int64_t, int, bool, R, hipStream_t, int);    \
+    template int fft_pass_z_c2r<R>(const cplx<R>*, R*, int, int64_t, R, int, const R*, double*, hipStream_t, int); \
     template int fft3d_r2c<R>(const R*, cplx<R>*, int, int64_t, hipStream_t);                                \
     template int fft3d_c2r<R>(const cplx<R>*, R*, cplx<R>*, int, int64_t, R, hipStream_t);
 PYL_INST(float)

@@ -22,17 +22,19 @@ template <typename R>
 int fft_pass_z_r2c(const R* in, cplx<R>* out, int N, int64_t P, bool affine, const double* dmass, double mass,
                    hipStream_t st);
 
-// column pass along y (axis 1) or x (axis 0) on a half spectrum, src -> dst (may alias)
+// column pass along y (axis 1) or x (axis 0) on a half spectrum, src -> dst (may alias).  K > 0
+// (axis 1 only): transform just the columns kz < K and leave the others untouched.
 template <typename R>
 int fft_pass_cols(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, bool inverse, R scale,
-                  hipStream_t st);
+                  hipStream_t st, int K = 0);
 
 // pass Z inverse: rows c2r (unnormalised * scale).  store: 0 = nothing written, 1 = x,
 // 2 = x * aux.  When acc != nullptr the kernel also adds sum(x^2) into acc[0] and sum(x * aux)
-// into acc[1] (float64; aux == nullptr counts as zero).
+// into acc[1] (float64; aux == nullptr counts as zero).  K > 0: entries kz >= K are zero and
+// are not read.
 template <typename R>
 int fft_pass_z_c2r(const cplx<R>* in, R* out, int N, int64_t P, R scale, int store, const R* aux,
-                   double* acc, hipStream_t st);
+                   double* acc, hipStream_t st, int K = 0);
 
 // full transforms
 template <typename R>
