@@ -206,8 +206,9 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
 
 namespace pyl {
 
-// float64 sum of the particle weights (the mass that sets rhobar, Pk_snapshot.py:80-84)
-__global__ __launch_bounds__(256) void k_weight_sum(const float* __restrict__ W, long long n, double* __restrict__ out) {
+// float64 sum of the particle weights (the mass that sets rhobar, Pk_snapshot.py:80-84): one
+// partial per workgroup, summed by sum_parts
+__global__ __launch_bounds__(256) void k_weight_sum(const float* __restrict__ W, long long n, double* __restrict__ part) {
     __shared__ double red[4];
     double a = 0.0;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -215,18 +216,19 @@ __global__ __launch_bounds__(256) void k_weight_sum(const float* __restrict__ W,
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 int weight_sum(const float* W, int64_t n, double* out, hipStream_t st) {
-    int rc = zero_fill(out, sizeof(double), st);
-    if (rc) return rc;
     long long blocks = (n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_weight_sum, dim3((unsigned)blocks), dim3(256), 0, st, W, (long long)n, out);
+    int rc = OK;
+    double* part = (double*)workspace(WS_RED, sizeof(double) * (size_t)blocks, &rc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_weight_sum, dim3((unsigned)blocks), dim3(256), 0, st, W, (long long)n, part);
     PYL_HIP(hipGetLastError());
-    return OK;
+    return sum_parts(part, (int)blocks, out, st);
 }
 
 // pos_redshift_space in place (redshift_space_library.pyx:29-43): one thread per particle,

@@ -147,7 +147,7 @@ __global__ void k_rows_sum(const double* __restrict__ part, int nblocks, int n, 
 // ------------------------------------------------------------------ filters
 // Top-hat: 1 where d^2 <= R^2 (cell units); Gaussian: exp(-d^2 / (2 R^2)) (float64, stored float32)
 __global__ __launch_bounds__(256) void k_filter(float* __restrict__ out, int N, float R2, int gaussian,
-                                                double* __restrict__ norm) {
+                                                double* __restrict__ part) {
     __shared__ double red[4];
     const int middle = N / 2;
     const long long total = (long long)N * N * N;
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_filter(float* __restrict__ out, int N, 
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(norm, (red[0] + red[1]) + (red[2] + red[3]));
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);   // summed by sum_parts
 }
 
 __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ f, long long n, const double* __restrict__ norm) {
@@ -334,13 +334,16 @@ extern "C" int pyl_ft_filter(int dims, float BoxSize, float R, int gaussian, voi
     if (rc) return rc;
     double* norm = (double*)workspace(WS_MASS, 64, &rc);
     if (rc) return rc;
-    rc = zero_fill(norm, sizeof(double), st);
+    const unsigned fb = grid_for((long long)ncell);
+    double* part = (double*)workspace(WS_RED, sizeof(double) * fb, &rc);
     if (rc) return rc;
     // R_grid = R*dims/BoxSize and R^2 in float32 (smoothing_library.pyx:34-35)
     const float Rg = (R * (float)dims) / BoxSize;
     const float R2 = Rg * Rg;
-    hipLaunchKernelGGL(k_filter, dim3(grid_for((long long)ncell)), dim3(256), 0, st, f, dims, R2, gaussian, norm);
+    hipLaunchKernelGGL(k_filter, dim3(fb), dim3(256), 0, st, f, dims, R2, gaussian, part);
     PYL_HIP(hipGetLastError());
+    rc = sum_parts(part, (int)fb, norm, st);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_normalise, dim3(grid_for((long long)ncell)), dim3(256), 0, st, f, (long long)ncell, norm);
     PYL_HIP(hipGetLastError());
     return fft3d_r2c<float>(f, (cplx<float>*)out_k, dims, half_len(dims), st);

@@ -42,6 +42,27 @@ int zero_fill(void* ptr, size_t bytes, hipStream_t st) {
     return OK;
 }
 
+__global__ __launch_bounds__(1024) void k_sum_parts(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    __shared__ double red[16];
+    const int tid = threadIdx.x;
+    double a = 0.0;
+    for (int b = tid; b < nb; b += 1024) a += part[b];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = a;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += red[w];
+        out[0] = s;
+    }
+}
+
+int sum_parts(const double* part, int nb, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, part, nb, out);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
 // Grow-only device buffers keyed by (device, slot).  Growing synchronises the device before
 // freeing the old buffer, so work already queued on it finishes first.
 void* workspace(int slot, size_t bytes, int* rc) {

@@ -6,10 +6,15 @@ namespace pyl {
 
 int zero_fill(void* ptr, size_t bytes, hipStream_t st);
 
+// out[0] = float64 sum of part[0 .. nb) by one workgroup (the second step of per-workgroup
+// partial sums, in place of one same-address atomic per workgroup)
+int sum_parts(const double* part, int nb, double* out, hipStream_t st);
+
 // Library-owned grow-only device workspace, one buffer per (device, slot).
 enum WsSlot { WS_SPEC = 0, WS_SPEC2 = 1, WS_SUMS = 2, WS_GRID = 3, WS_BINS = 4, WS_AUX = 5, WS_AUX2 = 6,
               WS_AUX3 = 7, WS_HOST_POS = 8, WS_HOST_GRID = 9, WS_HOST_W = 10, WS_OUT = 11, WS_MASS = 12,
-              WS_PART = 13, WS_STAGE = 14, WS_SUP = 15, WS_C2R = 16 };
+              WS_PART = 13, WS_STAGE = 14, WS_SUP = 15, WS_C2R = 16,
+              WS_RED = 17 };
 void* workspace(int slot, size_t bytes, int* rc);
 
 }  // namespace pyl
