@@ -170,8 +170,8 @@ __global__ __launch_bounds__(256) void k_grad_spec(cplx<R>* __restrict__ spec, i
         const int yy = (int)(t % N), xx = (int)(t / N);
         if (kzz > middle) continue;
         const int kx = kfreq(xx, N), ky = kfreq(yy, N), kz = kzz;
-        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
-        const int bin = isqrt_i(k2);
+        const int k2 = kx * kx + ky * ky + kz * kz;   // <= 3 (N/2)^2 < 2^24
+        const int bin = isqrt24(k2);
         double g = 0.0;
         if (bin >= 1 && bin <= nb) g = w[bin - 1] * unit / (double)n3[bin];
         const double c = win[xx] * win[yy] * win[kzz];

@@ -57,6 +57,16 @@ __device__ __forceinline__ int isqrt_i(long long v) {
     return (int)r;
 }
 
+// floor(sqrt(v)) for 0 <= v < 2^24 (float estimate within one of the root, then corrected)
+__device__ __forceinline__ int isqrt24(int v) {
+    int r = (int)__builtin_amdgcn_sqrtf((float)v);
+    r -= (r * r > v) ? 1 : 0;
+    r += ((r + 1) * (r + 1) <= v) ? 1 : 0;
+    return r;
+}
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // signed wavenumber of FFT index i on a grid of n points (Pk_library.pyx:315)
 __device__ __forceinline__ int kfreq(int i, int n) { return i > n / 2 ? i - n : i; }
 

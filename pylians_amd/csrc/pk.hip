@@ -216,15 +216,6 @@ template <typename R, int N> constexpr int nl_bin() {
 }
 template <int N, int NL> constexpr int threads_bin() { return threads_bin_c(N, NL); }
 
-// floor(sqrt(v)) for 0 <= v < 2^24 (float estimate within one of the root, then corrected)
-__device__ __forceinline__ int isqrt24(int v) {
-    int r = (int)__builtin_amdgcn_sqrtf((float)v);
-    r -= (r * r > v) ? 1 : 0;
-    r += ((r + 1) * (r + 1) <= v) ? 1 : 0;
-    return r;
-}
-__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
 // Squared phase atan2(re, |v|)^2 of a mode (Pk_library.pyx:374), r2 = |v|^2 >= re^2.  float32:
 // t = |re|/|v| in [0, 1], atan(t) = pi/4 + atan((t-1)/(t+1)) above tan(pi/8), then atan(x) =

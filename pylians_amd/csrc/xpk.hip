@@ -9,6 +9,7 @@
 // so its 2-D window stays in LDS.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/pylians_hip.h"
@@ -28,6 +29,8 @@ struct XBinArgs {
     const double* win[XMAXF];
     int ca[XMAXC], cb[XMAXC];
     int nc;
+    int nf, fl[XMAXF];       // distinct fields of the launch's pairs
+    int pa[XMAXC], pb[XMAXC];  // pair c = (fl[pa[c]], fl[pb[c]])
     long long P;
     int N, axis, last, fixk, nchunk, kmax_par, kmax_per, n1, n3, n2;
     double* s3;  // [nc][n3][3]
@@ -35,7 +38,8 @@ struct XBinArgs {
     double* s2;  // [nc][n2]
 };
 
-template <typename R, int NL, int T>
+// NF: register slots for the distinct fields of one launch (>= their number); NC: its pairs
+template <typename R, int NL, int T, int NF, int NC>
 __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
     constexpr int TPC = T / NL;
     extern __shared__ double hs[];
@@ -70,6 +74,16 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
         kzb1 = nkzb;
     }
     const int kpar_u = a.fixk ? 0 : abs(kfreq(u0, N));
+    // distinct fields of this launch's pairs (from the host): each is loaded once per mode
+    const int nf = a.nf;
+    const cplx<R>* sp[NF];
+    const double* wp[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int ff = f < nf ? a.fl[f] : a.fl[0];
+        sp[f] = (const cplx<R>*)a.spec[ff];
+        wp[f] = a.win[ff];
+    }
     for (int u = u0; u < u1; ++u)
         for (int kzb = kzb0; kzb < kzb1; ++kzb) {
             const int kz = kzb * NL + l;
@@ -77,72 +91,115 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
             const long long base = (long long)u * UST + kz;
             const int ku = kfreq(u, N);
             const bool zplane = (kz == 0) || (kz == middle && even);
+            // window factors of the two fixed coordinates (u and kz), per field
+            double wuz[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f < nf) wuz[f] = wp[f][u];
             int cur3 = -1, cur2 = -1;
-            double acc[XMAXC][3], b1[XMAXC], b2[XMAXC];
-            for (int c = 0; c < XMAXC; ++c) {
+            double acc[NC][3], b1[NC], b2[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
                 acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
                 b1[c] = b2[c] = 0.0;
             }
-            for (int q = 0; q < E; ++q) {
-                const int j = jrow * E + q;
-                if (j >= N) break;
-                const int kj = kfreq(j, N);
-                const int kx = a.last == 0 ? kj : ku;
-                const int ky = a.last == 0 ? ku : kj;
-                if (zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0))) continue;
-                const int xx = a.last == 0 ? j : u, yy = a.last == 0 ? u : j;
-                const int k2 = kx * kx + ky * ky + kz * kz;
-                const int bin = isqrt_i(k2);
-                const int kps = a.axis == 0 ? kx : (a.axis == 1 ? ky : kz);
-                const R mu2 = k2 == 0 ? (R)0 : (R)(kps * kps) / (R)k2;
-                const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
-                const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
-                R val[XMAXC];
+            // QB modes per batch: every field value and window entry of the batch is loaded before
+            // any of them is used
+            constexpr int QB = 4;
+            for (int q0 = 0; q0 < E; q0 += QB) {
+                cplx<R> v[QB][NF];
+                double wj[QB][NF];
 #pragma unroll
-                for (int c = 0; c < XMAXC; ++c) {
-                    if (c >= nc) break;
-                    const int fa = a.ca[c], fb = a.cb[c];
-                    const double* wa = a.win[fa];
-                    const double* wb = a.win[fb];
-                    const cplx<R> va = scale(((const cplx<R>*)a.spec[fa])[base + j * EST],
-                                             (R)(wa[xx] * wa[yy] * wa[kz]));
-                    const cplx<R> vb = fa == fb ? va : scale(((const cplx<R>*)a.spec[fb])[base + j * EST],
-                                                             (R)(wb[xx] * wb[yy] * wb[kz]));
-                    val[c] = va.x * vb.x + va.y * vb.y;
+                for (int b = 0; b < QB; ++b) {
+                    const int j = jrow * E + q0 + b;
+                    const bool in = q0 + b < E && j < N;
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+                        if (f < nf) {
+                            v[b][f] = in ? sp[f][base + j * EST] : mk<R>(0, 0);
+                            wj[b][f] = in ? wp[f][j] : 0.0;
+                        }
                 }
-                if (bin != cur3) {
-                    if (cur3 >= 0)
-                        for (int c = 0; c < nc; ++c)
-                            for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
-                    cur3 = bin;
-                    for (int c = 0; c < XMAXC; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
+#pragma unroll
+                for (int b = 0; b < QB; ++b) {
+                    const int j = jrow * E + q0 + b;
+                    if (q0 + b >= E || j >= N) break;
+                    const int kj = kfreq(j, N);
+                    const int kx = a.last == 0 ? kj : ku;
+                    const int ky = a.last == 0 ? ku : kj;
+                    if (zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0))) continue;
+                    const int k2 = kx * kx + ky * ky + kz * kz;
+                    const int bin = isqrt24(k2);   // |k|^2 <= 3 (N/2)^2 < 2^24 on every supported grid
+                    const int kps = a.axis == 0 ? kx : (a.axis == 1 ? ky : kz);
+                    const R mu2 = k2 == 0 ? (R)0 : (R)(kps * kps) * recip((R)k2);
+                    const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
+                    const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
+                    // each field's value times its window W(x) W(y) W(z), the product taken in the
+                    // order wa[x] * wa[y] * wa[z] (Pk_library.pyx:688-697)
+                    cplx<R> w[NF];
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+                        if (f < nf) {
+                            const double* wf = wp[f];
+                            const double wx = a.last == 0 ? wj[b][f] : wuz[f];
+                            const double wy = a.last == 0 ? wuz[f] : wj[b][f];
+                            w[f] = scale(v[b][f], (R)(wx * wy * wf[kz]));
+                        }
+                    R val[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        cplx<R> va = w[0], vb = w[0];
+#pragma unroll
+                        for (int f = 1; f < NF; ++f) {
+                            if (a.pa[c] == f) va = w[f];
+                            if (a.pb[c] == f) vb = w[f];
+                        }
+                        val[c] = va.x * vb.x + va.y * vb.y;
+                    }
+                    if (bin != cur3) {
+                        if (cur3 >= 0) {
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
+                        }
+                        cur3 = bin;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        acc[c][0] += (double)val[c];
+                        acc[c][1] += (double)(val[c] * L2);
+                        acc[c][2] += (double)(val[c] * L4);
+                    }
+                    if (k2 <= middle * middle) {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) b1[c] += (double)val[c];
+                    }
+                    const int kper = isqrt24(a.axis == 0 ? ky * ky + kz * kz
+                                                         : (a.axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
+                    const int w2 = (a.fixk ? l * nper : 0) + kper;
+                    if (w2 != cur2) {
+                        if (cur2 >= 0) {
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) atomicAdd(&h2[c * wlen + cur2], b2[c]);
+                        }
+                        cur2 = w2;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) b2[c] = 0.0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) b2[c] += (double)val[c];
                 }
-                for (int c = 0; c < nc; ++c) {
-                    acc[c][0] += (double)val[c];
-                    acc[c][1] += (double)(val[c] * L2);
-                    acc[c][2] += (double)(val[c] * L4);
-                }
-                if (k2 <= middle * middle)
-                    for (int c = 0; c < nc; ++c) b1[c] += (double)val[c];
-                const int kper = isqrt_i(a.axis == 0 ? ky * ky + kz * kz
-                                                     : (a.axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
-                const int w2 = (a.fixk ? l * nper : 0) + kper;
-                if (w2 != cur2) {
-                    if (cur2 >= 0)
-                        for (int c = 0; c < nc; ++c) atomicAdd(&h2[c * wlen + cur2], b2[c]);
-                    cur2 = w2;
-                    for (int c = 0; c < XMAXC; ++c) b2[c] = 0.0;
-                }
-                for (int c = 0; c < nc; ++c) b2[c] += (double)val[c];
             }
-            if (cur3 >= 0)
-                for (int c = 0; c < nc; ++c)
-                    for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
-            if (cur2 >= 0)
-                for (int c = 0; c < nc; ++c) atomicAdd(&h2[c * wlen + cur2], b2[c]);
             const int kpc = a.fixk ? kz : kpar_u;
-            for (int c = 0; c < nc; ++c)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (cur3 >= 0)
+                    for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
+                if (cur2 >= 0) atomicAdd(&h2[c * wlen + cur2], b2[c]);
                 if (b1[c] != 0.0) atomicAdd(&h1[c * a.n1 + kpc], b1[c]);
+            }
         }
     __syncthreads();
     for (int i = tid; i < a.n3 * nc * 3; i += T) {
@@ -158,6 +215,29 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
         const int kpar = a.fixk ? kzb0 * NL + r / nper : kpar_u;
         if (kpar <= a.kmax_par) atomicAdd(&a.s2[(long long)c * a.n2 + (long long)(a.kmax_par + 1) * kper + kpar], h2[i]);
     }
+}
+
+template <typename R, int NL, int T, int NF>
+static void launch_xbin_nc(const XBinArgs& a, int grid, size_t shm, hipStream_t st) {
+    switch (a.nc) {
+        case 1: hipLaunchKernelGGL((k_xbin<R, NL, T, NF, 1>), dim3((unsigned)grid), dim3(T), shm, st, a); break;
+        case 2: hipLaunchKernelGGL((k_xbin<R, NL, T, NF, 2>), dim3((unsigned)grid), dim3(T), shm, st, a); break;
+        case 3: hipLaunchKernelGGL((k_xbin<R, NL, T, NF, 3>), dim3((unsigned)grid), dim3(T), shm, st, a); break;
+        default: hipLaunchKernelGGL((k_xbin<R, NL, T, NF, 4>), dim3((unsigned)grid), dim3(T), shm, st, a); break;
+    }
+}
+
+template <typename R, int NL, int T>
+static int launch_xbin(const XBinArgs& a, int grid, size_t shm, hipStream_t st) {
+    PYL_CHECK(a.nc >= 1 && a.nc <= XMAXC && a.nf >= 1 && a.nf <= XMAXF, E_ARG, "bad pair group");
+    if (a.nf <= 2)
+        launch_xbin_nc<R, NL, T, 2>(a, grid, shm, st);
+    else if (a.nf <= 4)
+        launch_xbin_nc<R, NL, T, 4>(a, grid, shm, st);
+    else
+        launch_xbin_nc<R, NL, T, 8>(a, grid, shm, st);
+    PYL_HIP(hipGetLastError());
+    return OK;
 }
 
 static int num_cus_x() {
@@ -241,8 +321,22 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
         a.s1 = base + (size_t)3 * d.n3() * a.nc;
         a.s2 = a.s1 + (size_t)d.n1() * a.nc;
         const size_t shm = sizeof(double) * (size_t)a.nc * (3 * d.n3() + d.n1() + wlen);
-        hipLaunchKernelGGL((k_xbin<R, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, a);
-        PYL_HIP(hipGetLastError());
+        std::vector<int> fs;   // distinct fields of the launch
+        for (int c = 0; c < a.nc; ++c) {
+            int ip[2];
+            for (int e = 0; e < 2; ++e) {
+                const int f = e == 0 ? a.ca[c] : a.cb[c];
+                auto it = std::find(fs.begin(), fs.end(), f);
+                ip[e] = (int)(it - fs.begin());
+                if (it == fs.end()) fs.push_back(f);
+            }
+            a.pa[c] = ip[0];
+            a.pb[c] = ip[1];
+        }
+        a.nf = (int)fs.size();
+        for (int f = 0; f < a.nf; ++f) a.fl[f] = fs[f];
+        rc = launch_xbin<R, NL, T>(a, grid, shm, st);
+        if (rc) return rc;
     }
     std::vector<double> h((size_t)per * C);
     PYL_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
