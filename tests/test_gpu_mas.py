@@ -217,3 +217,14 @@ def test_presorted_clustered_and_edges(gpu):
             o = C.deposit_f64(p, dims, L, name)
             g = _dev_deposit(p, None, dims, L, scheme, _lib.SORTED | _lib.ZERO)
             assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, name
+
+
+def test_presorted_tiny_inputs(gpu):
+    """Presort with fewer particles than one workgroup step, and with a single particle."""
+    from pylians_amd import _lib
+    dims, L = 32, 10.0
+    for n in (1, 5, 700):
+        pos = np.random.default_rng(n).random((n, 3), dtype=np.float32) * np.float32(L)
+        o = C.deposit_f64(pos, dims, L, "TSC")
+        g = _dev_deposit(pos, None, dims, L, 3, _lib.SORTED | _lib.ZERO)
+        assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, n

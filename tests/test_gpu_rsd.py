@@ -105,3 +105,20 @@ def test_presorted_fused_rsd_matches_binned(gpu):
     for name in ("Pk", "Pk1D", "Pkphase"):
         assert np.allclose(getattr(g, name), getattr(r, name), rtol=1e-9, atol=0), name
     assert np.allclose(g.Pk2D[1:], r.Pk2D[1:], rtol=1e-9, atol=0)
+
+
+def test_pos_redshift_space_edges(gpu):
+    """No particles; an axis outside the columns; mismatched shapes."""
+    import torch
+    from pylians_amd import _lib
+    from pylians_amd import redshift_space_library as RSL
+    RSL.pos_redshift_space(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), 100.0, 70.0, 0.0, 1)
+    with pytest.raises(IndexError):
+        RSL.pos_redshift_space(np.zeros((4, 2), np.float32), np.zeros((4, 2), np.float32), 100.0, 70.0, 0.0, 2)
+    with pytest.raises(ValueError):
+        RSL.pos_redshift_space(np.zeros((4, 3), np.float32), np.zeros((5, 3), np.float32), 100.0, 70.0, 0.0, 0)
+    p = torch.zeros((4, 3), device="cuda")
+    rc = _lib.lib().pyl_pos_redshift_space(_lib.ptr(p), _lib.ptr(p), 4, 3, 100.0, 70.0, 0.0, 3, _lib.stream())
+    assert rc == _lib.E_ARG
+    rc = _lib.lib().pyl_pos_redshift_space(_lib.ptr(p), _lib.ptr(p), 4, 3, 100.0, 0.0, 0.0, 0, _lib.stream())
+    assert rc == _lib.E_ARG

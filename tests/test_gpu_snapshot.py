@@ -153,3 +153,19 @@ def test_pk_gadget_two_species(gpu, tmp_path, do_RSD):
     o = oracle_pk(tot, L, axis, "CIC")
     _check_table(str(tmp_path / ("Pk_CDM+NU" + suffix)), (o.k3D, o.Pk, o.Nmodes3D),
                  C.f32_noise_pk(tot, L, axis, "CIC").Pk)
+
+
+def test_pk_gadget_single_species_is_pk_comp(gpu, tmp_path):
+    """Pk_Gadget with one particle type writes Pk_comp's table (Pk_snapshot.py:113-117); two runs of
+    the path agree to the summation order of the float64 shell sums."""
+    from pylians_amd import Pk_snapshot
+    base, L, z, a, H, P, V, M, massarr = _two_species_snapshot(tmp_path)
+    out1, out2 = tmp_path / "a", tmp_path / "b"
+    out1.mkdir()
+    out2.mkdir()
+    assert Pk_snapshot.Pk_Gadget(base, 32, [1], False, 0, 1, str(out1)) is None
+    Pk_snapshot.Pk_comp(base, 1, 32, False, 0, 1, str(out2))
+    name = "Pk_CDM_z=%.3f.dat" % z
+    t1, t2 = np.loadtxt(out1 / name), np.loadtxt(out2 / name)
+    assert np.array_equal(t1[:, [0, 4]], t2[:, [0, 4]])
+    assert np.allclose(t1, t2, rtol=1e-12, atol=1e-12 * np.abs(t2[:, 1]).max())
