@@ -61,3 +61,30 @@ def test_autotuner_cache_roundtrip(gpu, tmp_path):
     path = tuning.save(str(tmp_path / "tune.json"))
     assert tuning.load(path) == len(ent)
     assert tuning.entries() == ent
+
+
+def test_graph_batch_presorted_weighted(gpu):
+    """Weighted jobs through the presorted deposit inside the captured graph (its count, scan and
+    scatter kernels and the zero fill are graph nodes): every replay equals the eager result, and
+    the result the FP64 oracle."""
+    from pylians_amd import _lib, batch, pipeline
+    jobs = _jobs()[:3]
+    rng = np.random.default_rng(7)
+    for j in jobs:
+        j["W"] = (rng.random(len(j["pos"])) + 0.25).astype(np.float32)
+    b = batch.PkBatch(jobs, flags=_lib.PK_ALL | _lib.SORTED)
+    eager = [pipeline.PkParticles(j["pos"], j["BoxSize"], j["dims"], MAS="CIC", axis=j["axis"], W=j["W"],
+                                  deposit="presorted") for j in jobs]
+    for rep in range(3):
+        b.run()
+        res = b.results()
+        x = torch.randn(1 << 20, device="cuda")
+        torch.cuda.synchronize()
+        assert float((x * 0).sum()) == 0.0
+        for g, e in zip(res, eager):
+            assert np.array_equal(g.Nmodes3D, e.Nmodes3D)
+            assert np.allclose(g.Pk, e.Pk, rtol=1e-11, atol=0)
+    for g, j in zip(res, jobs):
+        o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], W=j["W"], with_noise=True)
+        assert_pk_close(g, o, rtol=1e-5, noise=fl)
+    b.close()
