@@ -28,11 +28,12 @@ namespace pyl {
 
 constexpr int TILE = 16;
 constexpr int BT = 512;    // k_bin: threads per workgroup
-constexpr int KP = 2;      // k_bin: particles per thread per step
+constexpr int KP = 3;      // k_bin: particles per thread per step
 constexpr int HT = 2048;   // k_bin: LDS hash slots (> BT * KP, so an insertion always finds a slot)
 constexpr int TT = 512;    // k_tile: threads per workgroup
 constexpr int XCDS = 8;    // XCDs of gfx950; workgroup i runs on XCD i % XCDS
 static_assert(HT % BT == 0, "hash slots per thread");
+static_assert(HT > BT * KP, "every key of a step must find a hash slot");
 
 // Lowest stencil cell (unwrapped) and the float64 1-D weights; the same arithmetic as the
 // stencils of deposit.hip (MAS_library.pyx:151-157, 290, 393-399, 486-492).
