@@ -587,6 +587,7 @@ __global__ __launch_bounds__(PT) void k_sup_scatter(SupArgs a) {
         for (int k = 0; k < PKP; ++k) {
             if (sp[k] < 0) continue;
             const long long d = (long long)lb[sp[k]] + rank[k];
+            if (d >= a.n) continue;   // unreachable: the count pass gave every particle its slot
             a.spos[3 * d] = x[k];
             a.spos[3 * d + 1] = y[k];
             a.spos[3 * d + 2] = z[k];
