@@ -572,7 +572,7 @@ static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int ax
         nchunk = std::min(N, std::max(1, want / nkzb));
         grid = nkzb * nchunk;
     } else {
-        nchunk = std::min(nkzb, std::max(1, want / N));
+        nchunk = std::min(nkzb, std::max(1, (want + N - 1) / N));   // at least `want` blocks when it can
         grid = N * nchunk;
     }
     const int nper = d.kmax_per + 1;
