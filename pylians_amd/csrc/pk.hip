@@ -206,7 +206,10 @@ constexpr int hist_bytes(int N, int NL) {
 }
 // lines per block for the binning kernel: at most 8, and few enough that two blocks share a CU
 // (160 KB of LDS) beside their histograms
-constexpr int threads_bin_c(int N, int NL) { return clamp_i(N * NL / 16, 64, 512); }
+// (lines with a radix-3 factor from 192 points up: twice the threads, measured faster)
+constexpr int threads_bin_c(int N, int NL) {
+    return clamp_i(N * NL / 16 * (N % 3 == 0 && N >= 192 ? 2 : 1), 64, 512);
+}
 template <typename R, int N> constexpr int nl_bin() {
     int nl = 8;
     while (nl > 2 && (N + threads_bin_c(N, nl) / nl) * nl * (int)sizeof(cplx<R>) +

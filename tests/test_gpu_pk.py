@@ -116,3 +116,13 @@ def test_xpk_parity(PKL, axis):
         e2 = np.abs(g.PkX2D[:, x] - o.PkX2D[:, x]) / s2
         a2 = np.maximum(1e-5, NOISE_Z * _envelope(nz.PkX2D[:, x] / s2, n2shape))
         assert np.all((e2 <= a2)[1:]), ("cross 2D", x)
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_pk_float32_parity_192(PKL, axis):
+    """A 3-smooth grid large enough for the radix-3 launch shapes of pass Z and of the binning pass."""
+    N, L = 192, 600.0
+    delta = density_field(N, L, seed=7 + axis)
+    g = PKL.Pk(delta, L, axis=axis, MAS="CIC")
+    o = oracle_pk(delta, L, axis, "CIC")
+    assert_pk_close(g, o, rtol=1e-5, noise=f32_noise(delta, L, axis, "CIC"))
