@@ -126,3 +126,29 @@ def test_pk_float32_parity_192(PKL, axis):
     g = PKL.Pk(delta, L, axis=axis, MAS="CIC")
     o = oracle_pk(delta, L, axis, "CIC")
     assert_pk_close(g, o, rtol=1e-5, noise=f32_noise(delta, L, axis, "CIC"))
+
+
+def test_xpk_five_fields(PKL):
+    """Five fields: launch groups whose pairs touch more than four fields (the widest register
+    layout of the binning kernel); every auto and cross multipole against the float64 XPk."""
+    from oracle import pk as P
+    N, L = 32, 200.0
+    ds = [density_field(N, L, seed=40 + i, MAS="CIC") for i in range(5)]
+    MAS = ["CIC", "TSC", "CIC", "PCS", "NGP"]
+    g = PKL.XPk(ds, L, axis=2, MAS=MAS)
+    o = P.XPk([d.astype(np.float64) for d in ds], L, axis=2, MAS=MAS)
+    nz = C.f32_noise_xpk(ds, L, 2, MAS)
+    assert np.array_equal(g.Nmodes3D, o.Nmodes3D)
+
+    def ok(err, std):
+        return np.all(err <= np.maximum(1e-5, NOISE_Z * _envelope(std)))
+
+    for c in range(5):
+        P0 = np.abs(o.Pk[:, 0, c])
+        for l in range(3):
+            assert ok(np.abs(g.Pk[:, l, c] - o.Pk[:, l, c]) / P0, nz.Pk[:, l, c] / P0), ("auto", c, l)
+    pairs = [(a, b) for a in range(5) for b in range(a + 1, 5)]
+    for x, (a, b) in enumerate(pairs):
+        scale = np.sqrt(np.abs(o.Pk[:, 0, a] * o.Pk[:, 0, b]))
+        for l in range(3):
+            assert ok(np.abs(g.XPk[:, l, x] - o.XPk[:, l, x]) / scale, nz.XPk[:, l, x] / scale), ("cross", x, l)
