@@ -50,3 +50,22 @@ def test_bk_cached_indicator_sums_reused(PKL):
         assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= 1e-5
         outs.append(g.B)
     assert not np.allclose(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_bk_shells_past_nyquist(PKL, prec):
+    """Shells reaching beyond k_N (k3 up to 20.5 k_F on a 32^3 grid): every kz column can hold shell
+    modes, so the inverse passes run over the whole half spectrum."""
+    N, L = 32, 400.0
+    pos = synth.zeldovich(N, L, seed=77, rms_disp=6.0)
+    rho = C.deposit_f64(pos, N, L, "CIC")
+    delta = (rho / rho.mean() - 1.0).astype(np.float32 if prec == "f32" else np.float64)
+    kF = 2 * np.pi / L
+    k1, k2 = 8.2 * kF, 12.3 * kF
+    theta = np.linspace(0, np.pi, 5)
+    g = PKL.Bk(delta, L, k1, k2, theta, MAS="CIC")
+    o = Bo.Bk(delta.astype(np.float64), L, k1, k2, theta, MAS="CIC")
+    tol = 1e-5 if prec == "f32" else 1e-10
+    assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= tol
+    assert np.max(np.abs(g.B - o.B) / np.abs(o.B)) <= tol, (g.B, o.B)
+    assert np.max(np.abs(g.Q - o.Q) / np.abs(o.Q)) <= tol
