@@ -368,7 +368,7 @@ static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P
                            const R* aux, double* acc, hipStream_t st, int K) {
     constexpr int M = N / 2;
     constexpr int NL = Geo<R>::NL;
-    constexpr int T = threads_for(M, NL);
+    constexpr int T = threads_for(M, NL) * ((M % 3 == 0 && M >= 96) ? 2 : 1);   // as pass Z forward
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
     const long long nb = (nrows + NL - 1) / NL;
