@@ -40,28 +40,54 @@ def measured_traffic(workload):
     return out
 
 
-def algorithmic_bytes(n_p, N, S=2, tile=16):
-    """Per-kernel bytes each launch must move: inputs read once, outputs written once.
-
-    Float32 path, no weights.  The cell-binned deposit writes one 12-byte record per particle into
-    its (tile, halo class) bucket segment (k_bin); each 16^3 tile then reads its own records and the
-    records of its lower neighbours whose stencils reach it ((1 + (S-1)/16)^3 records per particle
-    on average) and writes its cells once (k_tile).
-    """
+def e2e_bytes(n_p, N, weights=False):
+    """Algorithmic bytes of one particles -> P(k) job: each stage's compulsory input read once and its
+    materialised output written once; fusion is credited through time, never through fewer bytes.
+      B_MA  = 12 N_p (+4 N_p with weights) + 4 N_g    (positions in, float32 overdensity grid out)
+      B_FFT = 4 N_g + 8 N_h                           (real grid in, complex64 half spectrum out)
+      B_BIN = 8 N_h                                   (half spectrum in; histograms negligible)"""
     Ng = N ** 3
     Nh = N * N * (N // 2 + 1)
-    halo = (1.0 + (S - 1) / tile) ** 3
+    return (12 + (4 if weights else 0)) * n_p + 4 * Ng + (4 * Ng + 8 * Nh) + 8 * Nh
+
+
+def algorithmic_bytes(n_p, N):
+    """Per-kernel algorithmic bytes (float32 path, no weights), the split of e2e_bytes over the
+    kernels that run it.  The deposit's compulsory traffic is its position read (k_bin: the bucket
+    records it writes and k_tile reads back are not algorithmic) and its grid write (k_tile); the
+    FFT's is the real grid read and the half spectrum written (pass Z); the binning reads the half
+    spectrum once (the last FFT pass, fused with it).  The middle FFT pass re-reads and re-writes
+    the half spectrum: an intermediate of the transform, counted with no algorithmic bytes of its
+    own in the end-to-end figure; its own line uses the 16 N_h it moves."""
+    Ng = N ** 3
+    Nh = N * N * (N // 2 + 1)
     return {
-        "deposit_atomic": 12 * n_p + 4 * Ng,     # read positions, add into the float32 grid
-        "deposit_bin": 24 * n_p,                 # read positions, write bucket records
-        "deposit_tiles": int(12 * n_p * halo) + 4 * Ng,
-        "grid_zero": 4 * Ng,
+        "deposit_atomic": 12 * n_p + 4 * Ng,     # read positions, write the grid
+        "deposit_bin": 12 * n_p,                 # read positions
+        "deposit_tiles": 4 * Ng,                 # write the overdensity grid
         "fft_z_r2c": 4 * Ng + 8 * Nh,            # read the real grid, write the half spectrum
-        "fft_mid": 16 * Nh,                       # read + write the half spectrum
-        "fft_last_bin": 8 * Nh,                   # read the half spectrum (histograms negligible)
-        # whole job: positions in, overdensity grid out, grid in, spectrum out/in twice, spectrum in
-        "e2e": (12 * n_p + 4 * Ng) + (4 * Ng + 8 * Nh) + 16 * Nh + 8 * Nh,
+        "fft_mid": 16 * Nh,                      # (intermediate: read + write the half spectrum)
+        "fft_last_bin": 8 * Nh,                  # read the half spectrum
+        "e2e": e2e_bytes(n_p, N),
     }
+
+
+def adjoint_bytes(n_p, N):
+    """Forward pass (e2e_bytes) plus the adjoint: gradient spectrum in and real gradient field out of
+    the c2r (8 N_h + 4 N_g), then positions and the field read, dL/dpos written by the gather
+    (12 N_p + 4 N_g + 12 N_p)."""
+    Ng = N ** 3
+    Nh = N * N * (N // 2 + 1)
+    return e2e_bytes(n_p, N) + (8 * Nh + 4 * Ng) + (24 * n_p + 4 * Ng)
+
+
+def bk_bytes(n_p, N, ntheta):
+    """One Bk realization with the indicator-field sums cached: the deposit (12 N_p + 4 N_g), the r2c
+    (4 N_g + 8 N_h), one c2r per shell (ntheta + 2 of them, 8 N_h + 4 N_g each) and one triple
+    product per theta bin (three float32 fields read, 12 N_g)."""
+    Ng = N ** 3
+    Nh = N * N * (N // 2 + 1)
+    return 12 * n_p + 4 * Ng + (4 * Ng + 8 * Nh) + (ntheta + 2) * (8 * Nh + 4 * Ng) + ntheta * 12 * Ng
 
 
 def cpu_leg(pos_host, N, L, axis, gpu_pk, threads):
@@ -129,17 +155,8 @@ def workload_batch(args):
     L in [250, 1000], uniform positions, CIC + P0, one hipGraph launch per step."""
     import torch
 
-    from pylians_amd import _lib, batch, synth
-    rng = np.random.default_rng(100)
-    jobs = []
-    for r in range(args.batch):
-        n_p = int(round(np.exp(rng.uniform(np.log(64 ** 3), np.log(256 ** 3)))))
-        N = int(rng.choice([128, 192, 256, 384]))
-        L = float(rng.uniform(250.0, 1000.0))
-        g = torch.Generator(device="cuda")
-        g.manual_seed(100 + r)
-        pos = torch.rand((n_p, 3), generator=g, device="cuda", dtype=torch.float32) * np.float32(L)
-        jobs.append({"pos": pos, "BoxSize": L, "dims": N, "MAS": "CIC", "axis": 0})
+    from pylians_amd import batch, synth
+    jobs = synth.batch_jobs(args.batch, seed=100)
     b = batch.PkBatch(jobs, flags=0)
     for _ in range(args.warmup):
         b.run()
@@ -152,13 +169,14 @@ def workload_batch(args):
         e1.record()
     b.stream.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
+    b.results()   # mode check of every job on the last replay
     n_tot = b.n_particles
-    cells = sum(j["dims"] ** 3 for j in jobs)
-    nh = sum(j["dims"] ** 2 * (j["dims"] // 2 + 1) for j in jobs)
-    bytes_e2e = 12 * n_tot + 8 * cells + 16 * nh
+    bytes_e2e = sum(e2e_bytes(int(j["pos"].shape[0]), j["dims"]) for j in jobs)
+    b.close()
     return {"metric": "Mparticles/s batched CIC+P0, %d uneven realizations in one hipGraph launch" % args.batch,
             "value": round(n_tot / (ms * 1e-3) / 1e6, 1), "unit": "Mparticles/s", "ms_per_step": round(ms, 4),
-            "grids_per_s": round(args.batch / (ms * 1e-3), 1), "e2e_hbm_frac": round(bytes_e2e / (ms * 1e-3) / 8e12, 4),
+            "grids_per_s": round(args.batch / (ms * 1e-3), 1), "bytes": bytes_e2e,
+            "e2e_hbm_frac": round(bytes_e2e / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
             "config": {"workload": "batch%d_uniform_cic_p0" % args.batch, "n_particles_total": n_tot,
                        "grids": sorted(set(j["dims"] for j in jobs))}, "dtype": "f32", "steps": args.steps}
 
@@ -168,42 +186,51 @@ def workload_adjoint(args):
     import torch
 
     from pylians_amd import autograd, synth
-    pos = synth.zeldovich_device(256, 500.0, seed=3, rms_disp=5.0)
+    N, L = 256, 500.0
+    pos = synth.zeldovich_device(256, L, seed=3, rms_disp=5.0)
     w = torch.tensor(np.random.default_rng(3).random(64), device="cuda")
-    ms = _timed(lambda: autograd.grad_loss(pos, 500.0, 256, w), args.steps, args.warmup)
+    ms = _timed(lambda: autograd.grad_loss(pos, L, N, w), args.steps, args.warmup)
     n = int(pos.shape[0])
+    nbytes = adjoint_bytes(n, N)
+    del pos
     return {"metric": "Mparticles/s forward P0 + adjoint dL/dpos, 256^3 CIC", "value": round(n / (ms * 1e-3) / 1e6, 1),
             "unit": "Mparticles/s", "ms_per_step": round(ms, 4), "dtype": "f32", "steps": args.steps,
-            "config": {"workload": "zeldovich256_cic_adjoint_64bins", "n_particles": n, "grid": 256, "BoxSize": 500.0}}
+            "bytes": nbytes, "e2e_hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "config": {"workload": "zeldovich256_cic_adjoint_64bins", "n_particles": n, "grid": N, "BoxSize": L}}
 
 
 def workload_bispectrum(args):
-    """16 realizations of 512^3 Zel'dovich particles on a 256^3 grid, CIC, Bk(k1=0.1, k2=0.2, 20 theta)."""
+    """Realizations of 512^3 Zel'dovich particles (resident in HBM), each: CIC onto a 256^3 grid,
+    delta = rho/mean - 1, Bk(k1=0.1, k2=0.2, 20 theta) -- the deposit is inside the timed region."""
     import torch
 
     from pylians_amd import Pk_library, _lib, synth
-    N, L = 256, 1000.0
-    theta = np.linspace(0, np.pi, 20)
-    deltas = []
-    for r in range(args.realizations):
-        pos = synth.zeldovich_device(512, L, seed=200 + r, rms_disp=5.0)
-        g = torch.zeros((N, N, N), dtype=torch.float32, device="cuda")
+    N, L, nth = 256, 1000.0, 20
+    theta = np.linspace(0, np.pi, nth)
+    grid = torch.empty((N, N, N), dtype=torch.float32, device="cuda")
+    parts = [synth.zeldovich_device(512, L, seed=200 + r, rms_disp=5.0) for r in range(args.realizations)]
+
+    def one(pos):
         _lib.check(_lib.lib().pyl_mas_deposit(_lib.ptr(pos), None, int(pos.shape[0]), 3, N, L, 2, _lib.ZERO,
-                                              _lib.ptr(g), _lib.stream()))
-        g /= g.double().mean().float()
-        g -= 1.0
-        deltas.append(g)
-        del pos
-    Pk_library.Bk(deltas[0], L, 0.1, 0.2, theta)       # builds the cached indicator sums
+                                              _lib.ptr(grid), _lib.stream()))
+        grid.div_(grid.double().mean().float())
+        grid.sub_(1.0)
+        return Pk_library.Bk(grid, L, 0.1, 0.2, theta)
+
+    one(parts[0])       # builds the cached indicator sums
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for d in deltas:
-        Pk_library.Bk(d, L, 0.1, 0.2, theta)
+    for pos in parts:
+        one(pos)
     torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / len(deltas)
-    return {"metric": "Bk per realization (20 theta bins, 256^3, indicator sums cached)", "value": round(ms, 3),
-            "unit": "ms/realization", "higher_is_better": False, "dtype": "f32",
-            "config": {"workload": "bk_zeldovich512_grid256_20theta", "realizations": len(deltas)}}
+    ms = (time.perf_counter() - t0) * 1e3 / len(parts)
+    n = int(parts[0].shape[0])
+    nbytes = bk_bytes(n, N, nth)
+    del parts
+    return {"metric": "Bk per realization (512^3 particles CIC onto 256^3, 20 theta bins, indicator sums cached)",
+            "value": round(ms, 3), "unit": "ms/realization", "higher_is_better": False, "dtype": "f32",
+            "bytes": nbytes, "e2e_hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "config": {"workload": "bk_zeldovich512_grid256_20theta", "realizations": args.realizations}}
 
 
 def main():
@@ -223,6 +250,8 @@ def main():
     ap.add_argument("--workload", default="zeldovich512", choices=["zeldovich512", "batch", "adjoint", "bispectrum"])
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--realizations", type=int, default=16)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload only: skip the batch, adjoint and bispectrum figures")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -255,7 +284,9 @@ def main():
     print("# generated %d particles in %.1fs" % (n_p, time.time() - t0), file=sys.stderr, flush=True)
     _, _, _, nout = pk_layout(N)
     out = torch.empty(nout, dtype=torch.float64, device="cuda")
-    flags = _lib.PK_ALL | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED}[args.deposit]
+    # PK_ASYNC: no host synchronisation inside the timed loop; the mode counts the kernel returns are
+    # checked after it (check_number_modes, as the reference does on its Nmodes3D)
+    flags = _lib.PK_ALL | _lib.PK_ASYNC | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED}[args.deposit]
 
     def step(prec):
         run_device(pos, None, out, L, N, scheme, axis, prec, flags)
@@ -284,6 +315,8 @@ def main():
 
     gpu_pk = R()
     unpack_pk(gpu_pk, res32, N)
+    from pylians_amd.Pk_library import check_number_modes
+    check_number_modes(np.concatenate([[1.0], gpu_pk.Nmodes3D]), N)
     print("# fp32c: %.3f ms/step (wall %.3f ms/step)" % (ms, wall * 1e3 / args.steps), file=sys.stderr, flush=True)
 
     # ---- plain float64 GPU path (same particles, float64 grid, FFT and sums)
@@ -297,6 +330,7 @@ def main():
     ms64 = ev0.elapsed_time(ev1) / args.fp64_steps
     res64 = R()
     unpack_pk(res64, out.cpu().numpy(), N)
+    check_number_modes(np.concatenate([[1.0], res64.Nmodes3D]), N)
     print("# fp64: %.3f ms/step" % ms64, file=sys.stderr, flush=True)
 
     # ---- host drop-in (numpy positions in, Pk attributes out: PCIe both ways); reported beside the
@@ -312,8 +346,8 @@ def main():
     host_ms = (time.perf_counter() - h0) * 1e3 / 3
     print("# host drop-in: %.2f ms/step" % host_ms, file=sys.stderr, flush=True)
 
-    # ---- roofline of the dominant kernel
-    ab = algorithmic_bytes(n_p, N, S=scheme)
+    # ---- roofline of the dominant kernel (the stage with the longest average launch)
+    ab = algorithmic_bytes(n_p, N)
     per = {k: (v[1] / v[0]) for k, v in stages.items() if v[0] > 0}
     dom = max((k for k in per if k in ab), key=lambda k: per[k])
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
@@ -322,8 +356,10 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
                 "bytes_per_launch": ab[dom], "avg_ms": round(per[dom], 4),
-                "traffic_by_stage": traffic or None}
+                "traffic_by_stage": traffic or None,
+                "traffic_over_algorithmic": {k: round(traffic[k] / ab[k], 3) for k in traffic if k in ab} or None}
     stage_ms = {k: round(v, 4) for k, v in per.items()}
+    stage_frac = {k: round(ab[k] / (per[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k in per if k in ab}
 
     line = {
         "metric": METRIC, "value": round(n_p / (ms * 1e-3) / 1e6, 1), "unit": "Mparticles/s", "n_gpus": 1,
@@ -335,12 +371,29 @@ def main():
                    "deposit": args.deposit or "auto"},
         "fp64_ms_per_step": round(ms64, 4), "speedup_fp32c_vs_fp64": round(ms64 / ms, 3),
         "grids_per_s": round(1e3 / ms, 3),
-        "e2e_hbm_frac": round(ab["e2e"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "stage_ms": stage_ms, "roofline": roofline,
+        "e2e_bytes": ab["e2e"], "e2e_hbm_frac": round(ab["e2e"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": stage_ms, "stage_hbm_frac": stage_frac, "roofline": roofline,
         "fp64_vs_fp32c_P0_maxrel": float(np.max(np.abs(res64.Pk[:, 0] - gpu_pk.Pk[:, 0]) / np.abs(res64.Pk[:, 0]))),
         "host_dropin": {"ms_per_step": round(host_ms, 2), "Mparticles_s": round(n_p / (host_ms * 1e-3) / 1e6, 1),
                         "note": "PkParticles on a pageable numpy array: positions host->device and results back"},
     }
+    del pos
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        # the other workloads of the benchmark set, each with its algorithmic bytes and HBM fraction
+        import copy
+        ex = copy.copy(args)
+        ex.steps, ex.warmup, ex.realizations = 5, 2, 4
+        extra = {}
+        for key, fn in (("batch64", workload_batch), ("adjoint256", workload_adjoint),
+                        ("bispectrum256", workload_bispectrum)):
+            t1 = time.time()
+            r = fn(ex)
+            extra[key] = {k: r[k] for k in ("value", "unit", "ms_per_step", "bytes", "e2e_hbm_frac", "config")
+                          if k in r}
+            print("# %s %.1fs" % (key, time.time() - t1), file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
+        line["workloads"] = extra
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         t1 = time.time()

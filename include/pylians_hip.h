@@ -55,6 +55,10 @@ extern "C" {
 #define PYL_PK_2D 2
 #define PYL_PK_PHASE 4
 #define PYL_PK_ALL 7
+#define PYL_PK_ASYNC 256     /* no host synchronisation for the mode check (see pyl_pk) */
+#define PYL_PK_DROP_KZ0 512  /* fault injection for tests of the mode check: the binning kernel leaves
+                                out the kz = 0 plane (a normal run that loses modes; never set by the
+                                library itself) */
 
 int pyl_abi_version(void);
 const char* pyl_last_error(void);
@@ -96,14 +100,19 @@ int pyl_pk_layout(int dims, int* kmax_par, int* kmax_per, int* kmax, int64_t* ou
 
 /* Replaces Pk_library.Pk (Pk_library.pyx:266-425).  delta: (N,N,N) real of precision prec.
  * mas_p: window exponent (0 = no deconvolution, as MAS='None').  deltak_out (nullable) receives
- * the corrected half spectrum (keep_deltak, Pk_library.pyx:423).  Returns PYL_E_MODES when the
- * independent-mode count is not (N^3-8)/2+8 (check_number_modes, Pk_library.pyx:90-102). */
+ * the corrected half spectrum (keep_deltak, Pk_library.pyx:423).
+ * Nmodes3D/1D/2D are the counts of the binning kernel itself (Pk_library.pyx:376-381).  Every bin's
+ * count is compared on the device with the independent-mode geometry; the call then synchronises
+ * `stream` and returns PYL_E_MODES when any bin differs -- which includes a total different from
+ * (N^3-8)/2+8 (check_number_modes, Pk_library.pyx:90-102, 411).  With PYL_PK_ASYNC (or inside a
+ * stream capture) the call does not synchronise and the caller checks the returned Nmodes. */
 int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags, double* out,
            void* deltak_out, void* stream);
 
 /* Fused end-to-end path of Pk_snapshot.Pk_comp (Pk_snapshot.py:80-91): deposit with `scheme`,
  * delta = rho/rhobar - 1 with rhobar = sum(W)/N^3 (N_p/N^3 without weights), r2c, window
- * deconvolution with the same scheme, binning.  The density grid is not returned. */
+ * deconvolution with the same scheme, binning.  The density grid is not returned.  The mode check
+ * and PYL_PK_ASYNC behave as in pyl_pk. */
 int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
                      int prec, int flags, double* out, void* stream);
 
@@ -171,9 +180,14 @@ int pyl_bk(const void* delta, int dims, double BoxSize, double k1, double k2, co
 /* ---------------------------------------------------------------- batches
  * Builder-defined (the reference processes one field per call, Pk/Pk.py:17-27).  A batch is an
  * array of independent particles -> P(k) jobs of any sizes, run back to back on one stream; the
- * jobs share the library workspaces.  pyl_graph_create runs the batch once (sizing every workspace
- * and plan), then captures it into a hipGraph; pyl_graph_launch replays the whole batch with one
- * launch.  Inputs are read and outputs written at the device addresses given in the jobs. */
+ * jobs share the library workspaces.  pyl_graph_create runs the batch once inside a private set of
+ * workspaces owned by the graph (sizing every buffer and plan), then captures it into a hipGraph;
+ * pyl_graph_launch replays the whole batch with one launch.  Because the graph owns its workspaces,
+ * eager calls made between replays (of any size) never invalidate it; launches of one graph must
+ * not overlap each other (replay it on one stream).  pyl_graph_destroy synchronises the device and
+ * frees the graph and its workspaces.  Inputs are read and outputs written at the device addresses
+ * given in the jobs.  Jobs run without the host-side mode check (PYL_PK_ASYNC is implied): the
+ * caller checks the Nmodes of each output. */
 typedef struct {
     const float* pos;   /* (n, 3) float32, device */
     const float* W;     /* (n,) float32 or NULL */

@@ -5,7 +5,8 @@
  *
  *  - oracle_deposit_f64: the NGP/CIC/TSC/PCS stencils of MAS_library.pyx:123-545
  *    (same weights and index rules as MAS_c.c:8-266) evaluated in float64 with a
- *    float64 grid, floor-based periodic indices.  Serial and deterministic.
+ *    float64 grid, floor-based periodic indices.  Serial and deterministic;
+ *    oracle_deposit_f64_par gives the same bits on several threads (grid planes split among them).
  *  - oracle_pkloop_f64: the Pk mode loop, Pk_library.pyx:314-381, in float64
  *    (float64 spectrum, float64 window factor).
  *  - oracle_pkloop_f32ref: the same loop as the reference compiles it: complex64
@@ -16,6 +17,9 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static inline long wrapi(long i, long n) { long r = i % n; return r < 0 ? r + n : r; }
 
@@ -70,6 +74,45 @@ void oracle_deposit_f64(const float *pos, const float *W, long n, int axes, int 
         } else {
             for (l = 0; l < S; l++) for (m = 0; m < S; m++)
                 grid[ix[0][l] * dims + ix[1][m]] += wx[0][l] * wx[1][m] * ww;
+        }
+    }
+}
+
+/* The same deposit on `threads` OpenMP threads, 3-D only.  Thread t owns the grid planes
+ * ix in [t*dims/threads, (t+1)*dims/threads) and walks every particle in input order, adding only
+ * the stencil cells of its planes: every cell receives exactly the serial function's terms in the
+ * serial order, so the result is bit-identical to oracle_deposit_f64. */
+void oracle_deposit_f64_par(const float *pos, const float *W, long n, int dims, double BoxSize,
+                            int scheme, double multiplicity, double *grid, int threads)
+{
+    double inv = dims / BoxSize;
+    if (threads < 1) threads = 1;
+    if (threads > dims) threads = dims;
+#pragma omp parallel num_threads(threads)
+    {
+        int t = 0, nt = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+        nt = omp_get_num_threads();
+#endif
+        long x0 = (long)dims * t / nt, x1 = (long)dims * (t + 1) / nt;
+        long ix[3][4]; double wx[3][4]; int S = 1, a, l, m, q;
+        long p;
+        for (p = 0; p < n; p++) {
+            double ww;
+            S = stencil((double)pos[p * 3] * inv, scheme, dims, ix[0], wx[0]);
+            for (l = 0; l < S; l++) if (ix[0][l] >= x0 && ix[0][l] < x1) break;
+            if (l == S) continue;
+            for (a = 1; a < 3; a++) S = stencil((double)pos[p * 3 + a] * inv, scheme, dims, ix[a], wx[a]);
+            ww = (W ? (double)W[p] : 1.0) * multiplicity;
+            for (l = 0; l < S; l++) {
+                if (ix[0][l] < x0 || ix[0][l] >= x1) continue;
+                for (m = 0; m < S; m++) {
+                    double wlm = wx[0][l] * wx[1][m] * ww;
+                    long row = (ix[0][l] * dims + ix[1][m]) * dims;
+                    for (q = 0; q < S; q++) grid[row + ix[2][q]] += wlm * wx[2][q];
+                }
+            }
         }
     }
 }

@@ -33,6 +33,9 @@ def lib():
         _lib.oracle_deposit_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        _lib.oracle_deposit_f64_par.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
+                                                ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                                ctypes.c_int]
         for f in (_lib.oracle_pkloop_f64, _lib.oracle_pkloop_f32ref):
             f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 9
     return _lib
@@ -56,11 +59,27 @@ def ref():
     return _ref
 
 
-def deposit_f64(pos, dims, BoxSize, scheme, W=None, multiplicity=1.0):
+def host_threads():
+    """Threads for the oracle's parallel loops: this process's CPU share, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def deposit_f64(pos, dims, BoxSize, scheme, W=None, multiplicity=1.0, threads=None):
+    """FP64 oracle deposit.  3-D inputs of more than 2^20 particles use the plane-split parallel
+    loop, whose result is bit-identical to the serial one."""
     pos = np.ascontiguousarray(pos, dtype=np.float32)
     n, axes = pos.shape
     grid = np.zeros((dims,) * axes, dtype=np.float64)
     Wc = None if W is None else np.ascontiguousarray(W, dtype=np.float32)
+    threads = host_threads() if threads is None else int(threads)
+    if axes == 3 and n > (1 << 20) and threads > 1:
+        lib().oracle_deposit_f64_par(_p(pos), None if Wc is None else _p(Wc), n, dims, float(BoxSize),
+                                     _SCHEME[scheme], float(multiplicity), _p(grid), threads)
+        return grid
     lib().oracle_deposit_f64(_p(pos), None if Wc is None else _p(Wc), n, axes, dims,
                              float(BoxSize), _SCHEME[scheme], float(multiplicity), _p(grid))
     return grid

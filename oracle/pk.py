@@ -22,6 +22,8 @@ of the full k-grid for Nmodes/k-means in 1-D/2-D/3-D; plane-wave closed forms
 and P4 = 3.375 P0 across it); the window closed form (x/sin x)^p; Parseval.
 """
 import math
+import os
+
 import numpy as np
 
 
@@ -61,16 +63,26 @@ def check_number_modes(Nmodes, dims):
         raise SystemExit("WARNING: Not all modes counted")
 
 
+def _workers():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return 1
+
+
 def FFT3Dr(a):
-    """Unnormalised forward r2c, (N,N,N) -> (N,N,N//2+1), complex128."""
-    return np.fft.rfftn(np.asarray(a, dtype=np.float64))
+    """Unnormalised forward r2c, (N,N,N) -> (N,N,N//2+1), complex128 (pocketfft in float64, the
+    library behind numpy.fft, run on the host's cores through scipy.fft)."""
+    import scipy.fft
+    return scipy.fft.rfftn(np.asarray(a, dtype=np.float64), workers=_workers())
 
 
 def IFFT3Dr(a, dims=None):
     """1/N^3-normalised c2r, (N,N,N//2+1) -> (N,N,N), float64."""
+    import scipy.fft
     a = np.asarray(a, dtype=np.complex128)
     n = a.shape[0] if dims is None else dims
-    return np.fft.irfftn(a, s=(n, n, n), axes=(0, 1, 2))
+    return scipy.fft.irfftn(a, s=(n, n, n), axes=(0, 1, 2), workers=_workers())
 
 
 def mode_grid(dims):

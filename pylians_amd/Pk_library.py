@@ -26,6 +26,8 @@ import torch
 from . import _lib
 
 VERBOSE = False
+# extra pyl_pk flags OR-ed into every Pk call; the tests of the mode check set PK_DROP_KZ0 here
+_EXTRA_FLAGS = 0
 
 __all__ = ["frequencies", "frequencies_2D", "MAS_function", "MAS_correction", "check_number_modes",
            "check_number_modes_2D", "FFT3Dr_f", "FFT3Dr_d", "IFFT3Dr_f", "IFFT3Dr_d", "Pk", "XPk", "Bk",
@@ -79,6 +81,18 @@ def check_number_modes_2D(Nmodes, dims):
     if int(np.sum(Nmodes)) != indep:
         print("WARNING: Not all modes counted")
         sys.exit()
+
+
+def _modes_exit(rc):
+    """check_number_modes' report and exit (Pk_library.pyx:98-102) for a PYL_E_MODES return."""
+    import re
+    msg = _lib.lib().pyl_last_error().decode(errors="replace")
+    m = re.search(r"counted (\d+) independent modes, expected (\d+)", msg)
+    print("WARNING: Not all modes counted")
+    if m:
+        print("Counted  %d independent modes" % int(m.group(1)))
+        print("Expected %d independent modes" % int(m.group(2)))
+    sys.exit()
 
 
 def _grid3(a, dtype, name):
@@ -178,11 +192,10 @@ class Pk:
         out = _lib.empty((nout,), torch.float64)
         dk = _lib.empty((dims, dims, dims // 2 + 1), cplx_t) if keep_deltak else None
         rc = _lib.lib().pyl_pk(_lib.ptr(d), dims, float(BoxSize), int(axis), MAS_function(MAS),
-                               _lib.F64 if f64 else _lib.F32, _lib.PK_ALL, _lib.ptr(out), _lib.ptr(dk),
-                               _lib.stream())
+                               _lib.F64 if f64 else _lib.F32, _lib.PK_ALL | _EXTRA_FLAGS, _lib.ptr(out),
+                               _lib.ptr(dk), _lib.stream())
         if rc == _lib.E_MODES:
-            print("WARNING: Not all modes counted")
-            sys.exit()
+            _modes_exit(rc)
         _lib.check(rc)
         unpack_pk(self, _lib.to_host(out), dims)
         if keep_deltak:
@@ -218,6 +231,8 @@ class XPk:
         out = np.zeros(n.value)
         rc = _lib.lib().pyl_xpk(ptrs, fields, dims, float(BoxSize), int(axis), mas, _lib.F64 if f64 else _lib.F32,
                                 out.ctypes.data_as(ctypes.c_void_p), _lib.stream())
+        if rc == _lib.E_MODES:
+            _modes_exit(rc)
         _lib.check(rc)
         kmax_par, kmax_per, kmax, _ = pk_layout(dims)
         n2 = (kmax_par + 1) * (kmax_per + 1)

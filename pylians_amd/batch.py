@@ -2,16 +2,21 @@
 
 ``PkBatch(jobs)`` takes a list of dicts ``{pos, BoxSize, dims, MAS='CIC', axis=0, W=None,
 precision='fp32'}`` with positions already on the GPU (or numpy arrays, uploaded once), runs the
-whole batch once to size the library workspaces, captures it into a single hipGraph and replays it
-with ``run()``; ``results()`` returns one ``Pk``-like object per job.  Builder-defined: the
-reference computes one field per call (Pk/Pk.py:17-27).
+whole batch once inside workspaces the graph owns, captures it into a single hipGraph and replays
+it with ``run()`` on the batch's own stream; ``results()`` returns one ``Pk``-like object per job,
+after the mode-count check of every job (check_number_modes, Pk_library.pyx:90-102, on the counts
+of the binning kernel).  Eager calls between replays do not disturb the graph (its workspaces are
+its own).  Without a graph, ``run()`` queues the jobs on the current torch stream, in the shared
+workspaces like any other call.  Builder-defined: the reference computes one field per call
+(Pk/Pk.py:17-27).
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
-from .Pk_library import pk_layout, unpack_pk
+from .Pk_library import check_number_modes, pk_layout, unpack_pk
 
 _PREC = {"fp32": _lib.F32, "fp64": _lib.F64}
 
@@ -48,12 +53,15 @@ class PkBatch:
             self._graph = h
 
     def run(self):
-        """Queue the whole batch (one graph launch) on this batch's stream."""
-        s = ctypes.c_void_p(self.stream.cuda_stream)
+        """Queue the whole batch: one graph launch on this batch's stream, or (without a graph) the
+        jobs one by one on the current stream."""
         if self._graph is not None:
+            s = ctypes.c_void_p(self.stream.cuda_stream)
             _lib.check(_lib.lib().pyl_graph_launch(self._graph, s))
         else:
-            _lib.check(_lib.lib().pyl_pk_batch(self._jobs, len(self._jobs), s))
+            self.stream.wait_stream(torch.cuda.current_stream())
+            _lib.check(_lib.lib().pyl_pk_batch(self._jobs, len(self._jobs), _lib.stream()))
+            self.stream.wait_stream(torch.cuda.current_stream())
 
     def results(self):
         self.stream.synchronize()
@@ -61,6 +69,8 @@ class PkBatch:
         for out, dims in self._outs:
             r = _Res()
             unpack_pk(r, _lib.to_host(out), dims)
+            # the DC mode is counted by the kernel and dropped from the output (Pk_library.pyx:411-412)
+            check_number_modes(np.concatenate([[1.0], r.Nmodes3D]), dims)
             res.append(r)
         return res
 

@@ -22,7 +22,9 @@ static_assert(PYL_GRID_F64 == DEP_GRID_F64 && PYL_DUP2D == DEP_DUP2D && PYL_RENO
                   PYL_ZERO == DEP_ZERO && PYL_ATOMIC == DEP_ATOMIC && PYL_BINNED == DEP_BINNED &&
                   PYL_SORTED == DEP_SORTED,
               "flag mismatch");
-static_assert(PYL_PK_1D == PK_1D && PYL_PK_2D == PK_2D && PYL_PK_PHASE == PK_PHASE, "flag mismatch");
+static_assert(PYL_PK_1D == PK_1D && PYL_PK_2D == PK_2D && PYL_PK_PHASE == PK_PHASE && PYL_PK_ASYNC == PK_ASYNC &&
+                  PYL_PK_DROP_KZ0 == PK_DROP_KZ0,
+              "flag mismatch");
 
 namespace pyl {
 int weight_sum(const float* W, int64_t n, double* out, hipStream_t st);
@@ -129,29 +131,42 @@ extern "C" int pyl_pk_layout(int dims, int* kmax_par, int* kmax_per, int* kmax, 
     return OK;
 }
 
-// float64 sums live in one workspace: s3 (4*n3) | s1 (n1) | s2 (n2) | mass (2)
-static int pk_sums_ws(const PkDims& d, PkSums* s, double** mass, hipStream_t st) {
+// float64 sums and mode counts live in one zeroed workspace (pk_sums_carve)
+static int pk_sums_ws(const PkDims& d, PkSums* s, hipStream_t st) {
     int rc = OK;
-    const size_t n = 4 * (size_t)d.n3() + d.n1() + d.n2() + 2;
+    const size_t n = pk_sums_doubles(d);
     double* b = (double*)workspace(WS_SUMS, sizeof(double) * n, &rc);
     if (rc) return rc;
     rc = zero_fill(b, sizeof(double) * n, st);
     if (rc) return rc;
-    s->s3 = b;
-    s->s1 = b + 4 * d.n3();
-    s->s2 = s->s1 + d.n1();
-    if (mass) *mass = s->s2 + d.n2();
+    pk_sums_carve(d, b, s);
     return OK;
 }
 
-static int check_modes(const PkGeometry* g) {
-    unsigned long long tot = 0;
-    for (auto v : g->h_n3) tot += v;
-    const long long N = g->d.N;
+// check_number_modes (Pk_library.pyx:90-102) on the counts of the binning kernel: pk_finalize
+// counted the bins whose count differs from the independent-mode geometry into sums.status.
+// Synchronises the stream unless PK_ASYNC is set or the stream is being captured (then the
+// counts are only in the output's Nmodes, for the caller to check).
+static int check_modes(const PkGeometry* g, const PkSums& sums, int flags, hipStream_t st) {
+    if (flags & PK_ASYNC) return OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PYL_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return OK;
+    unsigned bad = 0u;
+    PYL_HIP(hipMemcpyAsync(&bad, sums.status, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    PYL_HIP(hipStreamSynchronize(st));
+    if (bad == 0u) return OK;
+    const PkDims& d = g->d;
+    std::vector<double> c3(d.n3());
+    PYL_HIP(hipMemcpy(c3.data(), sums.c3, sizeof(double) * d.n3(), hipMemcpyDeviceToHost));
+    double tot = 0.0;
+    for (double v : c3) tot += v;
+    const long long N = d.N;
     const long long own = (N % 2 == 1) ? 1 : 8;
-    const unsigned long long expect = (unsigned long long)((N * N * N - own) / 2 + own);
-    PYL_CHECK(tot == expect, E_MODES, "Not all modes counted: counted %llu, expected %llu", tot, expect);
-    return OK;
+    const long long expect = (N * N * N - own) / 2 + own;
+    set_error("Not all modes counted: counted %.0f independent modes, expected %lld (%u bins differ from the "
+              "mode geometry)", tot, expect, bad);
+    return E_MODES;
 }
 
 template <typename R>
@@ -164,12 +179,10 @@ static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int
     const int64_t P = (half_len(dims) + ALIGN - 1) / ALIGN * ALIGN;
     const PkGeometry* g = pk_geometry(dims, axis, &rc);
     if (rc) return rc;
-    rc = check_modes(g);
-    if (rc) return rc;
     cplx<R>* spec = (cplx<R>*)workspace(WS_SPEC, sizeof(cplx<R>) * (size_t)dims * dims * P, &rc);
     if (rc) return rc;
     PkSums sums;
-    rc = pk_sums_ws(g->d, &sums, nullptr, st);
+    rc = pk_sums_ws(g->d, &sums, st);
     if (rc) return rc;
     {
         StageTimer t("fft_z_r2c", st);
@@ -187,8 +200,12 @@ static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int
         rc = pk_last_pass_bin<R>(spec, keep, dims, P, axis, mas_p, flags, sums, st);
         if (rc) return rc;
     }
-    StageTimer t("finalize", st);
-    return pk_finalize(g, sums, BoxSize, out, st);
+    {
+        StageTimer t("finalize", st);
+        rc = pk_finalize(g, sums, BoxSize, flags, out, st);
+        if (rc) return rc;
+    }
+    return check_modes(g, sums, flags, st);
 }
 
 extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags,
@@ -363,7 +380,7 @@ static int pk_particles(const float* pos, const float* W, int64_t n, int dims, d
     }
     rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, kind, grid, dmass, st, rsd);
     if (rc) return rc;
-    const int pkflags = flags & PYL_PK_ALL;
+    const int pkflags = flags & (PYL_PK_ALL | PYL_PK_ASYNC | PYL_PK_DROP_KZ0);
     if (f64)
         return pk_from_delta<double>((const double*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr,
                                      kind == DK_ATOMIC, dmass, (double)n, st);
@@ -423,7 +440,7 @@ extern "C" int pyl_pk_adjoint(const float* pos, int64_t n, int dims, double BoxS
         rc = deposit_delta(pos, nullptr, n, dims, BoxSize, scheme, false, kind, grid, nullptr, st);
         if (rc) return rc;
     }
-    rc = pk_from_delta<float>(grid, dims, BoxSize, 0, scheme, 0, out, keep, false, nullptr, (double)n, st);
+    rc = pk_from_delta<float>(grid, dims, BoxSize, 0, scheme, PK_ASYNC, out, keep, false, nullptr, (double)n, st);
     if (rc) return rc;
     const PkGeometry* g = pk_geometry(dims, 0, &rc);
     if (rc) return rc;
@@ -458,46 +475,73 @@ extern "C" int pyl_pk_batch(const pyl_pk_job* jobs, int njobs, void* stream) {
     PYL_CHECK(njobs >= 0 && (njobs == 0 || jobs), E_ARG, "bad job array");
     for (int i = 0; i < njobs; ++i) {
         const pyl_pk_job& j = jobs[i];
-        int rc = pyl_pk_particles(j.pos, j.W, j.n, j.dims, j.BoxSize, j.scheme, j.axis, j.prec, j.flags, j.out, stream);
+        int rc = pyl_pk_particles(j.pos, j.W, j.n, j.dims, j.BoxSize, j.scheme, j.axis, j.prec, j.flags | PYL_PK_ASYNC,
+                                  j.out, stream);
         if (rc) return rc;
     }
     return OK;
 }
 
 namespace {
+// A captured batch and the workspaces it owns: the graph's kernels hold pointers into the arena,
+// so later eager calls (which grow and free the shared workspaces) cannot invalidate them.
 struct GraphHandle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    WsArena* arena = nullptr;
 };
+
+void graph_free(GraphHandle* h) {
+    if (h->exec) (void)hipGraphExecDestroy(h->exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    ws_arena_destroy(h->arena);
+    delete h;
+}
 }  // namespace
 
 extern "C" int pyl_graph_create(const pyl_pk_job* jobs, int njobs, void* stream, void** graph) {
     PYL_CHECK(graph, E_ARG, "null graph handle");
     hipStream_t st = (hipStream_t)stream;
     PYL_CHECK(st != nullptr, E_ARG, "graph capture needs a non-default stream");
-    // warm run: sizes every workspace, builds twiddle / window / geometry tables
-    int rc = pyl_pk_batch(jobs, njobs, stream);
-    if (rc) return rc;
-    PYL_HIP(hipStreamSynchronize(st));
     GraphHandle* h = new GraphHandle();
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
-    if (e != hipSuccess) {
-        delete h;
-        PYL_HIP(e);
+    h->arena = ws_arena_create();
+    int rc = OK;
+    {
+        ArenaScope scope(h->arena);
+        // warm run: sizes every workspace of the arena, builds twiddle / window / geometry tables
+        // and settles the deposit choices
+        rc = pyl_pk_batch(jobs, njobs, stream);
+        hipError_t e = rc ? hipSuccess : hipStreamSynchronize(st);
+        if (rc == OK && e != hipSuccess) {
+            set_error("hipStreamSynchronize -> %s", hipGetErrorString(e));
+            rc = (int)e + 1000;
+        }
+        if (rc == OK) {
+            ws_arena_seal(h->arena);
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+            if (e != hipSuccess) {
+                set_error("hipStreamBeginCapture -> %s", hipGetErrorString(e));
+                rc = (int)e + 1000;
+            } else {
+                rc = pyl_pk_batch(jobs, njobs, stream);
+                e = hipStreamEndCapture(st, &h->graph);
+                if (rc == OK && e != hipSuccess) {
+                    set_error("hipStreamEndCapture -> %s", hipGetErrorString(e));
+                    rc = (int)e + 1000;
+                }
+            }
+        }
     }
-    rc = pyl_pk_batch(jobs, njobs, stream);
-    e = hipStreamEndCapture(st, &h->graph);
-    if (rc || e != hipSuccess) {
-        if (h->graph) (void)hipGraphDestroy(h->graph);
-        delete h;
-        if (rc) return rc;
-        PYL_HIP(e);
+    if (rc == OK) {
+        hipError_t e = hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            set_error("hipGraphInstantiate -> %s", hipGetErrorString(e));
+            rc = (int)e + 1000;
+        }
     }
-    e = hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(h->graph);
-        delete h;
-        PYL_HIP(e);
+    if (rc) {
+        graph_free(h);
+        return rc;
     }
     *graph = h;
     return OK;
@@ -511,9 +555,6 @@ extern "C" int pyl_graph_launch(void* graph, void* stream) {
 
 extern "C" int pyl_graph_destroy(void* graph) {
     if (!graph) return OK;
-    GraphHandle* h = (GraphHandle*)graph;
-    if (h->exec) PYL_HIP(hipGraphExecDestroy(h->exec));
-    if (h->graph) PYL_HIP(hipGraphDestroy(h->graph));
-    delete h;
+    graph_free((GraphHandle*)graph);
     return OK;
 }

@@ -98,6 +98,8 @@ __global__ __launch_bounds__(T) void k_c2r_rows(const cplx<R>* __restrict__ in, 
                                                  const R* __restrict__ aux, double* __restrict__ part, int K) {
     constexpr int N = 2 * M;
     constexpr int ES = NL + 1;
+    // whole wavefronts only: the float64 sums below reduce across all 64 lanes of every wave
+    static_assert(T % 64 == 0, "k_c2r_rows needs a multiple of 64 threads");
     __shared__ cplx<R> s[M * ES];
     __shared__ cplx<R> tw[N];
     __shared__ double red[2][T / 64];
@@ -368,7 +370,9 @@ static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P
                            const R* aux, double* acc, hipStream_t st, int K) {
     constexpr int M = N / 2;
     constexpr int NL = Geo<R>::NL;
-    constexpr int T = threads_for(M, NL) * ((M % 3 == 0 && M >= 96) ? 2 : 1);   // as pass Z forward
+    // as pass Z forward, rounded up to whole wavefronts (the fused sums reduce over full waves):
+    // M = 80 or 160 points would otherwise give 80 or 160 threads
+    constexpr int T = (threads_for(M, NL) * ((M % 3 == 0 && M >= 96) ? 2 : 1) + 63) / 64 * 64;
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
     const long long nb = (nrows + NL - 1) / NL;

@@ -199,10 +199,11 @@ constexpr int isqrt_c(long long v) {
     while ((r + 1) * (r + 1) <= v) ++r;
     return (int)r;
 }
-// LDS bytes of the histograms: 3-D (4 sums), 1-D, and the widest 2-D window (NL k_par rows)
+// LDS bytes of the histograms: 3-D (4 float64 sums + a mode count), 1-D and the widest 2-D
+// window (NL k_par rows), each a float64 sum and a 32-bit mode count
 constexpr int hist_bytes(int N, int NL) {
-    return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * 4 * 8 + (N / 2 + 1) * 8 +
-           NL * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 8;
+    return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * (4 * 8 + 4) + (N / 2 + 1) * 12 +
+           NL * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 12;
 }
 // lines per block for the binning kernel: at most 8, and few enough that two blocks share a CU
 // (160 KB of LDS) beside their histograms
@@ -304,6 +305,9 @@ __device__ __forceinline__ void last_stage_mapped(cplx<R>* s, const cplx<R>* tw,
 // its column blocks in order; the first-stage inputs of the next one are loaded into registers
 // while the current one is transformed and binned.  The spectrum has row pitch P (kz < W valid);
 // the optional window-corrected copy `keep` has pitch W.
+// Every histogram entry also counts the modes it received (Nmodes, Pk_library.pyx:376-381): a
+// block's row of `part` is [3-D sums (4 n3) | 1-D sums (n1) | 3-D counts (n3) | 1-D counts (n1) |
+// 2-D sums (n2w) | 2-D counts (n2w)], counts held as float64.
 template <typename R, int N, int NL, int T>
 __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
                                                  long long P, int W, int last, int axis, int fixk, int nchunk,
@@ -322,16 +326,22 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     __shared__ R win[N];
     extern __shared__ double hs[];
     const int nper = kmax_per + 1;
+    const int n2w = (fixk ? NL : 1) * nper;
+    const int nsum = 4 * n3 + n1 + n2w;
     double* h3 = hs;             // n3 * 4
     double* h1 = hs + 4 * n3;    // n1
     double* h2 = h1 + n1;        // NL * nper (fixk = 1) or nper (fixk = 0)
-    const int n2w = (fixk ? NL : 1) * nper;
+    unsigned* c3 = (unsigned*)(hs + nsum);   // mode counts: n3 | n1 | n2w
+    unsigned* c1 = c3 + n3;
+    unsigned* c2 = c1 + n1;
     const int tid = threadIdx.x;
     for (int i = tid; i < N; i += T) {
         tw[i] = twg[i];
         win[i] = (R)wing[i];
     }
-    for (int i = tid; i < 4 * n3 + n1 + n2w; i += T) hs[i] = 0.0;
+    for (int i = tid; i < nsum; i += T) hs[i] = 0.0;
+    for (int i = tid; i < n3 + n1 + n2w; i += T) c3[i] = 0u;
+    const bool drop0 = flags & PK_DROP_KZ0;
 
     const int middle = N / 2;
     const bool even = (N % 2) == 0;
@@ -402,7 +412,7 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
         }
         int u, kzz;
         item(it, u, kzz);
-        if (kzz >= W) continue;
+        if (kzz >= W || (drop0 && kzz == 0)) continue;
         const int kz = kzz;
         const int ku = kfreq(u, N);
         const R cuz = win[u] * win[kzz];
@@ -415,6 +425,7 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
         const long long kbase = (long long)u * KUST + kzz;
         int cur3 = -1, cur2 = -1;
         double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
+        unsigned m3 = 0u, m1 = 0u, m2 = 0u;   // modes of the current runs
         // pairs of rows with opposite k_j: the same |k|, k_perp, mu^2 and window factor
         for (int q = 0; q < EU; ++q) {
             const int uu = jrow * EU + q;
@@ -438,6 +449,7 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
                 if (pair) keep[kbase + jb * KEST] = ib ? scale(vb, fac) : vb;
             }
             if (!ia && !ib) continue;
+            const unsigned nm = (ia ? 1u : 0u) + (ib ? 1u : 0u);
             const R ra = va.x * va.x + va.y * va.y;
             const R rb = vb.x * vb.x + vb.y * vb.y;
             const R r2 = (ia ? ra : (R)0) + (ib ? rb : (R)0);
@@ -453,11 +465,14 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
                     atomicAdd(&h3[4 * cur3 + 1], a2);
                     atomicAdd(&h3[4 * cur3 + 2], a4);
                     if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+                    atomicAdd(&c3[cur3], m3);
                 }
                 cur3 = bin;
                 a0 = a2 = a4 = ap = 0.0;
+                m3 = 0u;
             }
             const double dd = (double)d2;
+            m3 += nm;
             a0 += dd;
             a2 += (double)(d2 * L2);
             a4 += (double)(d2 * L4);
@@ -468,15 +483,23 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
                 if (ib) pp += phase_sq(vb.x, rb);
                 ap += (double)pp;
             }
-            if (do1 && k2 <= middle * middle) b1 += dd;
+            if (do1 && k2 <= middle * middle) {
+                b1 += dd;
+                m1 += nm;
+            }
             if (do2) {
                 const int w2 = (fixk ? l * nper : 0) + isqrt24(cper + uu * uu);
                 if (w2 != cur2) {
-                    if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
+                    if (cur2 >= 0) {
+                        atomicAdd(&h2[cur2], b2);
+                        atomicAdd(&c2[cur2], m2);
+                    }
                     cur2 = w2;
                     b2 = 0.0;
+                    m2 = 0u;
                 }
                 b2 += dd;
+                m2 += nm;
             }
         }
         if (cur3 >= 0) {
@@ -484,25 +507,40 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
             atomicAdd(&h3[4 * cur3 + 1], a2);
             atomicAdd(&h3[4 * cur3 + 2], a4);
             if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
+            atomicAdd(&c3[cur3], m3);
         }
-        if (do1 && b1 != 0.0) atomicAdd(&h1[abs(kps)], b1);
-        if (cur2 >= 0) atomicAdd(&h2[cur2], b2);
+        if (m1 != 0u) {
+            atomicAdd(&h1[abs(kps)], b1);
+            atomicAdd(&c1[abs(kps)], m1);
+        }
+        if (cur2 >= 0) {
+            atomicAdd(&h2[cur2], b2);
+            atomicAdd(&c2[cur2], m2);
+        }
     }
     // this block's histograms -> its row of the partials (summed by k_bin_reduce31 / k_bin_reduce2)
     __syncthreads();
     double* row = part + (long long)blockIdx.x * RS;
-    for (int i = tid; i < 4 * n3 + n1 + n2w; i += T) row[i] = hs[i];
+    const int n31 = 5 * n3 + 2 * n1;
+    for (int i = tid; i < 4 * n3 + n1; i += T) row[i] = hs[i];
+    for (int i = tid; i < n3 + n1; i += T) row[4 * n3 + n1 + i] = (double)c3[i];
+    for (int i = tid; i < n2w; i += T) {
+        row[n31 + i] = h2[i];
+        row[n31 + n2w + i] = (double)c2[i];
+    }
 }
 
 // Sum the per-block histogram rows of k_last_bin into the pass sums.
 // 3-D and 1-D entries (k_bin_reduce31): 64 entries per workgroup, the blocks split over the 4 waves
-// and over gridDim.y slices; each slice adds its total with one atomic.
+// and over gridDim.y slices; each slice adds its total with one atomic.  Entries: 3-D sums, 1-D
+// sums, 3-D counts, 1-D counts.
 __global__ __launch_bounds__(256) void k_bin_reduce31(const double* __restrict__ part, int RS, int nblocks, int n3,
-                                                      int n1, double* __restrict__ g3, double* __restrict__ g1) {
+                                                      int n1, double* __restrict__ g3, double* __restrict__ g1,
+                                                      double* __restrict__ gc3, double* __restrict__ gc1) {
     __shared__ double red[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
-    const int n31 = 4 * n3 + n1;
+    const int n31 = 5 * n3 + 2 * n1;
     const int step = 4 * gridDim.y;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < n31) {
@@ -513,32 +551,46 @@ __global__ __launch_bounds__(256) void k_bin_reduce31(const double* __restrict__
     __syncthreads();
     if (w == 0 && i < n31) {
         const double t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-        if (t != 0.0) atomicAdd(i < 4 * n3 ? &g3[i] : &g1[i - 4 * n3], t);
+        double* dst = i < 4 * n3 ? &g3[i]
+                      : i < 4 * n3 + n1 ? &g1[i - 4 * n3]
+                      : i < 5 * n3 + n1 ? &gc3[i - 4 * n3 - n1]
+                                        : &gc1[i - 5 * n3 - n1];
+        if (t != 0.0) atomicAdd(dst, t);
     }
 }
 
 // 2-D entries (k_bin_reduce2): (k_per, k_par) comes from the blocks whose window holds k_par:
 // fixk = 0, the blocks of u = k_par and u = N - k_par (k_par = |k_u|); fixk = 1, the blocks of kz
 // block k_par / NL, row k_par % NL.
-__global__ void k_bin_reduce2(const double* __restrict__ part, int RS, int nblocks, int n31, int nper, int kmax_par,
-                              int fixk, int nchunk, int NL, int N, double* __restrict__ g2) {
+// The 2-D counts sit n2w entries after the 2-D sums of a row.
+__global__ void k_bin_reduce2(const double* __restrict__ part, int RS, int nblocks, int n31, int n2w, int nper,
+                              int kmax_par, int fixk, int nchunk, int NL, int N, double* __restrict__ g2,
+                              double* __restrict__ gc2) {
     const long long i2 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i2 >= (long long)(kmax_par + 1) * nper) return;
     const int kpar = (int)(i2 % (kmax_par + 1)), kper = (int)(i2 / (kmax_par + 1));
-    double a = 0.0;
+    double a = 0.0, m = 0.0;
     if (fixk) {
         const int kzb = kpar / NL, r = kpar % NL;
         for (int c = 0; c < nchunk; ++c) {
             const long long b = (long long)kzb * nchunk + c;
-            if (b < nblocks) a += part[b * RS + n31 + r * nper + kper];
+            if (b < nblocks) {
+                a += part[b * RS + n31 + r * nper + kper];
+                m += part[b * RS + n31 + n2w + r * nper + kper];
+            }
         }
     } else {
         const int us[2] = {kpar, N - kpar};
         const int nu = (kpar > 0 && N - kpar != kpar && N - kpar > N / 2) ? 2 : 1;
         for (int q = 0; q < nu; ++q)
-            for (int c = 0; c < nchunk; ++c) a += part[((long long)us[q] * nchunk + c) * RS + n31 + kper];
+            for (int c = 0; c < nchunk; ++c) {
+                const long long b = (long long)us[q] * nchunk + c;
+                a += part[b * RS + n31 + kper];
+                m += part[b * RS + n31 + n2w + kper];
+            }
     }
     if (a != 0.0) g2[i2] += a;
+    if (m != 0.0) gc2[i2] += m;
 }
 
 static int num_cus() {
@@ -579,22 +631,24 @@ static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int ax
         grid = N * nchunk;
     }
     const int nper = d.kmax_per + 1;
-    const int RS = 4 * d.n3() + d.n1() + (fixk ? NL : 1) * nper;
-    size_t shm = sizeof(double) * (size_t)RS;
+    const int n2w = (fixk ? NL : 1) * nper;
+    const int RS = 5 * d.n3() + 2 * d.n1() + 2 * n2w;
+    // LDS: float64 sums (4 n3 + n1 + n2w), then 32-bit counts (n3 + n1 + n2w)
+    size_t shm = sizeof(double) * (size_t)(4 * d.n3() + d.n1() + n2w) + sizeof(unsigned) * (size_t)(d.n3() + d.n1() + n2w);
     int rc = OK;
     double* part = (double*)workspace(WS_PART, sizeof(double) * (size_t)grid * RS, &rc);
     if (rc) return rc;
     hipLaunchKernelGGL((k_last_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P, W,
                        last, axis, fixk, nchunk, flags, d.kmax_par, d.kmax_per, d.n1(), d.n3(), win, tw, part, RS);
     PYL_HIP(hipGetLastError());
-    const int n31 = 4 * d.n3() + d.n1();
+    const int n31 = 5 * d.n3() + 2 * d.n1();
     const int slices = std::max(1, std::min(16, grid / 32));
     hipLaunchKernelGGL(k_bin_reduce31, dim3((unsigned)((n31 + 63) / 64), (unsigned)slices), dim3(256), 0, st, part, RS,
-                       grid, d.n3(), d.n1(), sums.s3, sums.s1);
+                       grid, d.n3(), d.n1(), sums.s3, sums.s1, sums.c3, sums.c1);
     PYL_HIP(hipGetLastError());
     const long long n2 = (long long)(d.kmax_par + 1) * nper;
-    hipLaunchKernelGGL(k_bin_reduce2, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, part, RS, grid, n31, nper,
-                       d.kmax_par, fixk, nchunk, NL, N, sums.s2);
+    hipLaunchKernelGGL(k_bin_reduce2, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, part, RS, grid, n31, n2w,
+                       nper, d.kmax_par, fixk, nchunk, NL, N, sums.s2, sums.c2);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -623,11 +677,29 @@ template int pk_last_pass_bin<double>(const cplx<double>*, cplx<double>*, int, i
                                       hipStream_t);
 
 // ------------------------------------------------------------------ finalize
-__global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxSize,
+size_t pk_sums_doubles(const PkDims& d) { return 5 * (size_t)d.n3() + 2 * (size_t)d.n1() + 2 * (size_t)d.n2() + 1; }
+
+void pk_sums_carve(const PkDims& d, double* b, PkSums* s) {
+    s->s3 = b;
+    s->s1 = s->s3 + 4 * d.n3();
+    s->s2 = s->s1 + d.n1();
+    s->c3 = s->s2 + d.n2();
+    s->c1 = s->c3 + d.n3();
+    s->c2 = s->c1 + d.n1();
+    s->status = (unsigned*)(s->c2 + d.n2());
+}
+
+// Nmodes of every histogram enabled in `flags` come from the binning kernel's counts and are
+// compared, bin by bin, with the geometry (the 3-D comparison includes the DC bin, so it contains
+// the total test of check_number_modes, Pk_library.pyx:90-102); histograms that were not binned
+// report the geometry.
+__global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxSize, int flags,
                               const unsigned long long* __restrict__ n1c, const unsigned long long* __restrict__ n2c,
                               const unsigned long long* __restrict__ n3c, const double* __restrict__ k3s,
                               const double* __restrict__ s3, const double* __restrict__ s1,
-                              const double* __restrict__ s2, double* __restrict__ out) {
+                              const double* __restrict__ s2, const double* __restrict__ c3,
+                              const double* __restrict__ c1, const double* __restrict__ c2,
+                              unsigned* __restrict__ status, double* __restrict__ out) {
     // evaluate as written (no fused multiply-add), as the reference's Python arithmetic does:
     // the last 1-D bin needs kN^2 - k^2 == 0 exactly
 #pragma clang fp contract(off)
@@ -648,40 +720,50 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
     double* kper = kpar + n2;
     double* P2D = kper + n2;
     double* N2 = P2D + n2;
+    const bool do1 = flags & PK_1D, do2 = flags & PK_2D;
     const int gi = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
-    for (int i = gi; i < kmax; i += gs) {
-        const double n = (double)n3c[i + 1];
-        k3D[i] = (k3s[i + 1] / n) * kF;
-        P0[i] = (s3[4 * (i + 1) + 0] / n) * unit;
-        P2[i] = (s3[4 * (i + 1) + 1] * 5.0 / n) * unit;
-        P4[i] = (s3[4 * (i + 1) + 2] * 9.0 / n) * unit;
-        ph[i] = (s3[4 * (i + 1) + 3] / n) * unit;
-        N3[i] = n;
+    unsigned bad = 0u;
+    for (int i = gi; i <= kmax; i += gs) {
+        const double n = c3[i];
+        bad += n != (double)n3c[i] ? 1u : 0u;
+        if (i == 0) continue;   // DC: counted and checked, not reported (Pk_library.pyx:412)
+        const int o = i - 1;
+        k3D[o] = (k3s[i] / n) * kF;
+        P0[o] = (s3[4 * i + 0] / n) * unit;
+        P2[o] = (s3[4 * i + 1] * 5.0 / n) * unit;
+        P4[o] = (s3[4 * i + 2] * 9.0 / n) * unit;
+        ph[o] = (s3[4 * i + 3] / n) * unit;
+        N3[o] = n;
     }
-    for (int i = gi; i < kmax_par; i += gs) {
-        const double n = (double)n1c[i + 1];
-        const double k = ((double)(i + 1) * n / n) * kF;
+    for (int i = gi; i <= kmax_par; i += gs) {
+        const double n = do1 ? c1[i] : (double)n1c[i];
+        if (do1) bad += n != (double)n1c[i] ? 1u : 0u;
+        if (i == 0) continue;
+        const int o = i - 1;
+        const double k = ((double)i * n / n) * kF;
         const double kmaxper = sqrt(kN * kN - k * k);
-        k1D[i] = k;
-        P1[i] = s1[i + 1] * unit * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
-        N1[i] = n;
+        k1D[o] = k;
+        P1[o] = s1[i] * unit * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
+        N1[o] = n;
     }
     for (int i = gi; i < n2; i += gs) {
         const int kp = i % (kmax_par + 1), kq = i / (kmax_par + 1);
-        const double n = (double)n2c[i];
+        const double n = do2 ? c2[i] : (double)n2c[i];
+        if (do2) bad += n != (double)n2c[i] ? 1u : 0u;
         kpar[i] = 0.5 * (kp + kp + 1) * kF;
         kper[i] = 0.5 * (kq + kq + 1) * kF;
         P2D[i] = s2[i] * unit / n;
         N2[i] = n;
     }
+    if (bad) atomicAdd(status, bad);
 }
 
-int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, double* out, hipStream_t st) {
+int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out, hipStream_t st) {
     const PkDims& d = g->d;
-    const int m = std::max(d.kmax, d.n2());
+    const int m = std::max(d.kmax + 1, d.n2());
     hipLaunchKernelGGL(k_pk_finalize, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, d.N, d.kmax, d.kmax_par,
-                       d.n2(), BoxSize, g->n1,
-                       g->n2, g->n3, g->k3sum, sums.s3, sums.s1, sums.s2, out);
+                       d.n2(), BoxSize, flags, g->n1, g->n2, g->n3, g->k3sum, sums.s3, sums.s1, sums.s2, sums.c3,
+                       sums.c1, sums.c2, sums.status, out);
     PYL_HIP(hipGetLastError());
     return OK;
 }

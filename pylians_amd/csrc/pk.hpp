@@ -30,14 +30,24 @@ struct PkGeometry {
 };
 const PkGeometry* pk_geometry(int N, int axis, int* rc);
 
-// float64 per-bin sums of the data-dependent quantities, device arrays owned by caller
+// float64 per-bin sums of the data-dependent quantities and the mode counts of the binning
+// kernel (integers held in float64, exact below 2^53), device arrays owned by caller
 struct PkSums {
     double* s3;   // n3 * 4: sum d2, sum d2 L2(mu), sum d2 L4(mu), sum phase^2
     double* s1;   // n1: sum d2 over modes with |k| <= N/2, binned by |k_par|
     double* s2;   // n2: sum d2 over 2-D bins (kmax_par+1)*k_per + k_par
+    double* c3;   // n3: modes binned per 3-D bin
+    double* c1;   // n1: modes binned per 1-D bin
+    double* c2;   // n2: modes binned per 2-D bin
+    unsigned* status;   // [1]: bins whose count differs from the mode geometry (set by pk_finalize)
 };
+size_t pk_sums_doubles(const PkDims& d);   // doubles of one PkSums block (status word included)
+void pk_sums_carve(const PkDims& d, double* base, PkSums* s);
 
-enum PkFlags { PK_1D = 1, PK_2D = 2, PK_PHASE = 4 };
+// PK_ASYNC: no host synchronisation for the mode check (the counts are still in the output).
+// PK_DROP_KZ0: the binning kernel leaves out the kz = 0 plane -- fault injection for the tests of
+// the mode check, never set by the library itself.
+enum PkFlags { PK_1D = 1, PK_2D = 2, PK_PHASE = 4, PK_ASYNC = 256, PK_DROP_KZ0 = 512 };
 
 // Axis of the last forward FFT pass for a line of sight `axis`: 1 (y) for axis 0, else 0 (x).
 // The half spectrum must have gone through pass Z and the column pass along the other axis.
@@ -48,11 +58,13 @@ template <typename R>
 int pk_last_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int axis, int mas_p, int flags,
                      const PkSums& sums, hipStream_t st);
 
-// Normalisation of Pk_library.pyx:387-421 on the device.  out layout (float64):
+// Normalisation of Pk_library.pyx:387-421 on the device, with Nmodes from the binning kernel's
+// counts; every bin whose count differs from the geometry is counted into sums.status (the
+// check_number_modes test, Pk_library.pyx:90-102, made per bin).  out layout (float64):
 //   [k3D (kmax) | P0 (kmax) | P2 (kmax) | P4 (kmax) | Pkphase (kmax) | Nmodes3D (kmax) |
 //    k1D (kmax_par) | Pk1D (kmax_par) | Nmodes1D (kmax_par) |
 //    kpar (n2) | kper (n2) | Pk2D (n2) | Nmodes2D (n2)]
 size_t pk_out_len(const PkDims& d);
-int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, double* out, hipStream_t st);
+int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out, hipStream_t st);
 
 }  // namespace pyl

@@ -36,6 +36,9 @@ bool profiling_enabled() { return g_on; }
 
 StageTimer::StageTimer(const char* n, hipStream_t s) : name(n), st(s) {
     if (!g_on) return;
+    // no events inside a stream capture: they would become graph nodes that are never resolved
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     std::lock_guard<std::mutex> lk(g_mu);
     e0 = take_event();
     if (e0 && hipEventRecord(e0, st) == hipSuccess) live = true;

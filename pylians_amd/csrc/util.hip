@@ -65,19 +65,14 @@ int sum_parts(const double* part, int nb, double* out, hipStream_t st) {
 
 // Grow-only device buffers keyed by (device, slot).  Growing synchronises the device before
 // freeing the old buffer, so work already queued on it finishes first.
-void* workspace(int slot, size_t bytes, int* rc) {
-    static std::mutex mu;
-    static std::map<std::pair<int, int>, std::pair<void*, size_t>> ws;
-    *rc = OK;
+struct WsArena {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) {
-        set_error("hipGetDevice failed");
-        *rc = E_LAUNCH;
-        return nullptr;
-    }
-    std::lock_guard<std::mutex> lk(mu);
-    auto& e = ws[std::make_pair(dev, slot)];
-    if (e.second >= bytes && e.first) return e.first;
+    bool sealed = false;
+    std::map<int, std::pair<void*, size_t>> slots;
+};
+static thread_local WsArena* t_arena = nullptr;
+
+static void* grow(std::pair<void*, size_t>& e, size_t bytes, int* rc) {
     if (e.first) {
         hipDeviceSynchronize();
         hipFree(e.first);
@@ -95,6 +90,68 @@ void* workspace(int slot, size_t bytes, int* rc) {
     return e.first;
 }
 
-void release_workspaces() {}
+void* workspace(int slot, size_t bytes, int* rc) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::pair<void*, size_t>> ws;
+    *rc = OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *rc = E_LAUNCH;
+        return nullptr;
+    }
+    if (WsArena* a = t_arena) {
+        if (a->dev != dev) {
+            set_error("graph workspace of device %d used on device %d", a->dev, dev);
+            *rc = E_ARG;
+            return nullptr;
+        }
+        auto& e = a->slots[slot];
+        if (e.second >= bytes && e.first) return e.first;
+        if (a->sealed) {
+            set_error("graph workspace slot %d would grow from %zu to %zu bytes during capture", slot, e.second,
+                      bytes);
+            *rc = E_ALLOC;
+            return nullptr;
+        }
+        return grow(e, bytes, rc);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto& e = ws[std::make_pair(dev, slot)];
+    if (e.second >= bytes && e.first) return e.first;
+    return grow(e, bytes, rc);
+}
+
+WsArena* ws_arena_create() {
+    WsArena* a = new WsArena();
+    if (hipGetDevice(&a->dev) != hipSuccess) a->dev = 0;
+    return a;
+}
+
+void ws_arena_seal(WsArena* a) {
+    if (a) a->sealed = true;
+}
+
+void ws_arena_destroy(WsArena* a) {
+    if (!a) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(a->dev);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : a->slots)
+        if (kv.second.first) (void)hipFree(kv.second.first);
+    (void)hipSetDevice(cur);
+    delete a;
+}
+
+size_t ws_arena_bytes(const WsArena* a) {
+    size_t s = 0;
+    if (a)
+        for (auto& kv : a->slots) s += kv.second.second;
+    return s;
+}
+
+ArenaScope::ArenaScope(WsArena* a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
 
 }  // namespace pyl

@@ -17,4 +17,21 @@ enum WsSlot { WS_SPEC = 0, WS_SPEC2 = 1, WS_SUMS = 2, WS_GRID = 3, WS_BINS = 4, 
               WS_RED = 17 };
 void* workspace(int slot, size_t bytes, int* rc);
 
+// A private set of workspaces (one buffer per slot) owned by a captured hipGraph.  While an
+// ArenaScope is alive on a thread, workspace() on that thread serves the arena's buffers instead of
+// the shared ones, so a graph keeps valid pointers whatever later eager calls do to the shared slots.
+// A sealed arena never grows (a capture must not reallocate a buffer an earlier node references):
+// a request beyond its size fails with E_ALLOC.
+struct WsArena;
+WsArena* ws_arena_create();
+void ws_arena_seal(WsArena* a);
+// frees every buffer (synchronises the device first); the graph using it must be destroyed before
+void ws_arena_destroy(WsArena* a);
+size_t ws_arena_bytes(const WsArena* a);
+struct ArenaScope {
+    explicit ArenaScope(WsArena* a);
+    ~ArenaScope();
+    WsArena* prev;
+};
+
 }  // namespace pyl

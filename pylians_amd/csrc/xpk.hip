@@ -36,6 +36,11 @@ struct XBinArgs {
     double* s3;  // [nc][n3][3]
     double* s1;  // [nc][n1]
     double* s2;  // [nc][n2]
+    // modes binned per 3-D / 1-D / 2-D bin (Pk_library.pyx:713-728), counted by the first launch
+    // only (every launch walks the same modes); nullptr in the others
+    double* gc3;  // [n3]
+    double* gc1;  // [n1]
+    double* gc2;  // [n2]
 };
 
 // NF: register slots for the distinct fields of one launch (>= their number); NC: its pairs
@@ -48,8 +53,14 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
     double* h3 = hs;                       // [n3][nc][3]
     double* h1 = h3 + a.n3 * nc * 3;       // [nc][n1]
     double* h2 = h1 + nc * a.n1;           // [nc][wlen]
+    const int nsum = a.n3 * nc * 3 + nc * a.n1 + nc * wlen;
+    unsigned* q3 = (unsigned*)(hs + nsum);   // mode counts [n3] | [n1] | [wlen]
+    unsigned* q1 = q3 + a.n3;
+    unsigned* q2 = q1 + a.n1;
+    const bool cnt = a.gc3 != nullptr;
     const int tid = threadIdx.x;
-    for (int i = tid; i < a.n3 * nc * 3 + nc * a.n1 + nc * wlen; i += T) hs[i] = 0.0;
+    for (int i = tid; i < nsum; i += T) hs[i] = 0.0;
+    for (int i = tid; i < a.n3 + a.n1 + wlen; i += T) q3[i] = 0u;
     __syncthreads();
     const int middle = N / 2;
     const bool even = (N % 2) == 0;
@@ -97,6 +108,7 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
             for (int f = 0; f < NF; ++f)
                 if (f < nf) wuz[f] = wp[f][u];
             int cur3 = -1, cur2 = -1;
+            unsigned m3 = 0u, m1 = 0u, m2 = 0u;   // modes of the current runs
             double acc[NC][3], b1[NC], b2[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -161,11 +173,14 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
 #pragma unroll
                             for (int c = 0; c < NC; ++c)
                                 for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
+                            if (cnt) atomicAdd(&q3[cur3], m3);
                         }
                         cur3 = bin;
+                        m3 = 0u;
 #pragma unroll
                         for (int c = 0; c < NC; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
                     }
+                    ++m3;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
                         acc[c][0] += (double)val[c];
@@ -173,6 +188,7 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
                         acc[c][2] += (double)(val[c] * L4);
                     }
                     if (k2 <= middle * middle) {
+                        ++m1;
 #pragma unroll
                         for (int c = 0; c < NC; ++c) b1[c] += (double)val[c];
                     }
@@ -183,11 +199,14 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
                         if (cur2 >= 0) {
 #pragma unroll
                             for (int c = 0; c < NC; ++c) atomicAdd(&h2[c * wlen + cur2], b2[c]);
+                            if (cnt) atomicAdd(&q2[cur2], m2);
                         }
                         cur2 = w2;
+                        m2 = 0u;
 #pragma unroll
                         for (int c = 0; c < NC; ++c) b2[c] = 0.0;
                     }
+                    ++m2;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) b2[c] += (double)val[c];
                 }
@@ -198,10 +217,27 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
                 if (cur3 >= 0)
                     for (int m = 0; m < 3; ++m) atomicAdd(&h3[(cur3 * nc + c) * 3 + m], acc[c][m]);
                 if (cur2 >= 0) atomicAdd(&h2[c * wlen + cur2], b2[c]);
-                if (b1[c] != 0.0) atomicAdd(&h1[c * a.n1 + kpc], b1[c]);
+                if (m1 != 0u) atomicAdd(&h1[c * a.n1 + kpc], b1[c]);
+            }
+            if (cnt) {
+                if (cur3 >= 0) atomicAdd(&q3[cur3], m3);
+                if (cur2 >= 0) atomicAdd(&q2[cur2], m2);
+                if (m1 != 0u) atomicAdd(&q1[kpc], m1);
             }
         }
     __syncthreads();
+    if (cnt) {
+        for (int i = tid; i < a.n3; i += T)
+            if (q3[i]) atomicAdd(&a.gc3[i], (double)q3[i]);
+        for (int i = tid; i < a.n1; i += T)
+            if (q1[i]) atomicAdd(&a.gc1[i], (double)q1[i]);
+        for (int i = tid; i < wlen; i += T) {
+            if (!q2[i]) continue;
+            const int kper = i % nper;
+            const int kpar = a.fixk ? kzb0 * NL + i / nper : kpar_u;
+            if (kpar <= a.kmax_par) atomicAdd(&a.gc2[(long long)(a.kmax_par + 1) * kper + kpar], (double)q2[i]);
+        }
+    }
     for (int i = tid; i < a.n3 * nc * 3; i += T) {
         if (h3[i] == 0.0) continue;
         const int bin = i / (nc * 3), c = (i / 3) % nc, m = i % 3;
@@ -276,10 +312,13 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
         }
     const int C = (int)pa.size();
     const int per = 3 * d.n3() + d.n1() + d.n2();
-    double* sums = (double*)workspace(WS_SUMS, sizeof(double) * (size_t)per * C, &rc);
+    // pair sums, then the mode counts n3 | n1 | n2
+    const size_t nsums = (size_t)per * C + d.n3() + d.n1() + d.n2();
+    double* sums = (double*)workspace(WS_SUMS, sizeof(double) * nsums, &rc);
     if (rc) return rc;
-    rc = zero_fill(sums, sizeof(double) * (size_t)per * C, st);
+    rc = zero_fill(sums, sizeof(double) * nsums, st);
     if (rc) return rc;
+    double* gcount = sums + (size_t)per * C;
     XBinArgs a{};
     for (int f = 0; f < F; ++f) {
         a.spec[f] = spec + nspec * f;
@@ -300,7 +339,9 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
     const int wlen = (a.fixk ? NL : 1) * nper;
     // pairs per launch: as many (<= 4) as fit the LDS histograms
     int cmax = XMAXC;
-    while (cmax > 1 && (size_t)cmax * (3 * d.n3() + d.n1() + wlen) * 8 > 120 * 1024) --cmax;
+    while (cmax > 1 && (size_t)cmax * (3 * d.n3() + d.n1() + wlen) * 8 + (size_t)(d.n3() + d.n1() + wlen) * 4 >
+                           120 * 1024)
+        --cmax;
     const int nkzb = (int)((P + NL - 1) / NL);
     a.nchunk = 1;
     int grid = N;
@@ -320,7 +361,11 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
         a.s3 = base;
         a.s1 = base + (size_t)3 * d.n3() * a.nc;
         a.s2 = a.s1 + (size_t)d.n1() * a.nc;
-        const size_t shm = sizeof(double) * (size_t)a.nc * (3 * d.n3() + d.n1() + wlen);
+        const size_t shm = sizeof(double) * (size_t)a.nc * (3 * d.n3() + d.n1() + wlen) +
+                           sizeof(unsigned) * (size_t)(d.n3() + d.n1() + wlen);
+        a.gc3 = c0 == 0 ? gcount : nullptr;
+        a.gc1 = c0 == 0 ? gcount + d.n3() : nullptr;
+        a.gc2 = c0 == 0 ? gcount + d.n3() + d.n1() : nullptr;
         std::vector<int> fs;   // distinct fields of the launch
         for (int c = 0; c < a.nc; ++c) {
             int ip[2];
@@ -338,29 +383,48 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
         rc = launch_xbin<R, NL, T>(a, grid, shm, st);
         if (rc) return rc;
     }
-    std::vector<double> h((size_t)per * C);
+    std::vector<double> h(nsums);
     PYL_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     PYL_HIP(hipStreamSynchronize(st));
+    // Nmodes from the kernel's counts, checked bin by bin against the independent-mode geometry
+    // (check_number_modes, Pk_library.pyx:90-102, 785)
+    const double* k3c = h.data() + (size_t)per * C;
+    const double* k1c = k3c + d.n3();
+    const double* k2c = k1c + d.n1();
+    {
+        int bad = 0;
+        double tot = 0.0;
+        for (int i = 0; i < d.n3(); ++i) {
+            bad += k3c[i] != (double)g->h_n3[i];
+            tot += k3c[i];
+        }
+        for (int i = 0; i < d.n1(); ++i) bad += k1c[i] != (double)g->h_n1[i];
+        for (int i = 0; i < d.n2(); ++i) bad += k2c[i] != (double)g->h_n2[i];
+        const long long NN = N, own = (N % 2 == 1) ? 1 : 8;
+        PYL_CHECK(bad == 0, E_MODES,
+                  "Not all modes counted: counted %.0f independent modes, expected %lld (%d bins differ from the "
+                  "mode geometry)", tot, (NN * NN * NN - own) / 2 + own, bad);
+    }
     // finalize (Pk_library.pyx:730-785)
     const double kF = 2.0 * M_PI / BoxSize, kN = d.middle * kF;
     const double u1 = BoxSize / ((double)N * N);
     const double fact = u1 * u1 * u1;
     const int kmax = d.kmax, kp = d.kmax_par, n2 = d.n2();
     double* o = out;
-    for (int i = 0; i < kmax; ++i) o[i] = g->h_k3sum[i + 1] / (double)g->h_n3[i + 1] * kF;
+    for (int i = 0; i < kmax; ++i) o[i] = g->h_k3sum[i + 1] / k3c[i + 1] * kF;
     o += kmax;
-    for (int i = 0; i < kmax; ++i) o[i] = (double)g->h_n3[i + 1];
+    for (int i = 0; i < kmax; ++i) o[i] = k3c[i + 1];
     o += kmax;
-    for (int i = 0; i < kp; ++i) o[i] = ((double)(i + 1) * g->h_n1[i + 1] / (double)g->h_n1[i + 1]) * kF;
+    for (int i = 0; i < kp; ++i) o[i] = ((double)(i + 1) * k1c[i + 1] / k1c[i + 1]) * kF;
     double* k1D = o;
     o += kp;
-    for (int i = 0; i < kp; ++i) o[i] = (double)g->h_n1[i + 1];
+    for (int i = 0; i < kp; ++i) o[i] = k1c[i + 1];
     o += kp;
     for (int i = 0; i < n2; ++i) o[i] = 0.5 * (2 * (i % (kp + 1)) + 1) * kF;
     o += n2;
     for (int i = 0; i < n2; ++i) o[i] = 0.5 * (2 * (i / (kp + 1)) + 1) * kF;
     o += n2;
-    for (int i = 0; i < n2; ++i) o[i] = (double)g->h_n2[i];
+    for (int i = 0; i < n2; ++i) o[i] = k2c[i];
     o += n2;
     for (int c = 0; c < C; ++c) {
         // locate this pair's sums inside its launch group
@@ -370,19 +434,19 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
         const double* s1 = gb + (size_t)3 * d.n3() * nc + (size_t)cc * d.n1();
         const double* s2 = gb + (size_t)3 * d.n3() * nc + (size_t)d.n1() * nc + (size_t)cc * n2;
         for (int i = 0; i < kmax; ++i) {
-            const double n = (double)g->h_n3[i + 1];
+            const double n = k3c[i + 1];
             o[3 * i + 0] = (s3[3 * (i + 1) + 0] / n) * fact;
             o[3 * i + 1] = (s3[3 * (i + 1) + 1] * 5.0 / n) * fact;
             o[3 * i + 2] = (s3[3 * (i + 1) + 2] * 9.0 / n) * fact;
         }
         o += 3 * kmax;
         for (int i = 0; i < kp; ++i) {
-            const double n = (double)g->h_n1[i + 1];
+            const double n = k1c[i + 1];
             const double kmaxper = sqrt(kN * kN - k1D[i] * k1D[i]);
             o[i] = s1[i + 1] * fact * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
         }
         o += kp;
-        for (int i = 0; i < n2; ++i) o[i] = s2[i] * fact / (double)g->h_n2[i];
+        for (int i = 0; i < n2; ++i) o[i] = s2[i] * fact / k2c[i];
         o += n2;
     }
     return OK;

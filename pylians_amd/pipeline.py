@@ -9,7 +9,8 @@ With ``vel``, the redshift-space shift of pos_redshift_space (redshift_space_lib
 is applied to each position as the deposit loads it.
 
 ``run_device`` is the allocation-free inner call used by the benchmark: device inputs, a
-device output vector, no host synchronisation.
+device output vector; with ``flags | PK_ASYNC`` no host synchronisation (the caller then checks
+the returned Nmodes, which the binning kernel counted).
 """
 import numpy as np
 import torch
@@ -59,5 +60,11 @@ class PkParticles:
         _, _, _, nout = pk_layout(dims)
         out = _lib.empty((nout,), torch.float64)
         f = flags | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED, "presorted": _lib.SORTED}[deposit]
-        run_device(p, w, out, BoxSize, dims, _lib.SCHEMES[MAS], axis, PRECISION[precision], f, rsd)
+        try:
+            run_device(p, w, out, BoxSize, dims, _lib.SCHEMES[MAS], axis, PRECISION[precision], f, rsd)
+        except _lib.PylError as e:
+            if e.code == _lib.E_MODES:
+                from .Pk_library import _modes_exit
+                _modes_exit(e.code)
+            raise
         unpack_pk(self, _lib.to_host(out), dims)

@@ -112,3 +112,23 @@ def zeldovich_device(n_side, BoxSize, seed, rms_disp=5.0, kc=0.5, sigma_v=0.0, a
         x = torch.where(out, torch.remainder(x + L, L), x)
         pos[:, axis] = x
     return pos
+
+
+def batch_jobs(count=64, seed=100, MAS="CIC", axis=0):
+    """The uneven batch workload: `count` realizations, realization r drawn from a PCG64 stream seeded
+    with `seed`: N_p = round(exp(U(ln 64^3, ln 256^3))), grid N uniform over {128, 192, 256, 384},
+    L ~ U(250, 1000) Mpc/h, positions uniform in [0, L) generated on the GPU (Philox stream seeded
+    with seed + r).  Returns PkBatch job dicts with device positions."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for r in range(count):
+        n_p = int(round(np.exp(rng.uniform(np.log(64 ** 3), np.log(256 ** 3)))))
+        N = int(rng.choice([128, 192, 256, 384]))
+        L = float(rng.uniform(250.0, 1000.0))
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed + r)
+        pos = torch.rand((n_p, 3), generator=g, device="cuda", dtype=torch.float32) * np.float32(L)
+        jobs.append({"pos": pos, "BoxSize": L, "dims": N, "MAS": MAS, "axis": axis})
+    return jobs

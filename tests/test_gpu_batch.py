@@ -88,3 +88,31 @@ def test_graph_batch_presorted_weighted(gpu):
         o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], W=j["W"], with_noise=True)
         assert_pk_close(g, o, rtol=1e-5, noise=fl)
     b.close()
+
+
+def test_graph_owns_its_workspaces(gpu):
+    """A captured batch keeps working after eager calls that grow (and so free and reallocate) the
+    library's shared workspaces, also when such a call runs on another stream while the graph
+    replays: the graph's buffers are its own."""
+    from pylians_amd import Pk_library, batch, pipeline
+    jobs = _jobs()[:3]
+    b = batch.PkBatch(jobs)
+    eager = [pipeline.PkParticles(j["pos"], j["BoxSize"], j["dims"], MAS="CIC", axis=j["axis"]) for j in jobs]
+    big_pos = synth.uniform(64 ** 3, 700.0, seed=31)
+    for rep, dims in enumerate((96, 128, 160)):
+        # larger than every job: the shared slots are freed and allocated again
+        big = pipeline.PkParticles(big_pos, 700.0, dims, MAS="PCS", axis=rep)
+        b.run()
+        # overlapping the replay (the graph runs on the batch's stream, this on the current one)
+        f = torch.randn((dims, dims, dims), device="cuda", dtype=torch.float32)
+        Pk_library.Pk(f, 700.0, axis=0, MAS="None")
+        res = b.results()
+        for g, e in zip(res, eager):
+            assert np.array_equal(g.Nmodes3D, e.Nmodes3D)
+            assert np.allclose(g.Pk, e.Pk, rtol=1e-12, atol=0)
+            assert np.allclose(g.Pk1D, e.Pk1D, rtol=1e-12, atol=1e-300)
+        assert int(big.Nmodes3D.sum()) + 1 == (dims ** 3 - 8) // 2 + 8
+    for g, j in zip(res, jobs):
+        o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_noise=True)
+        assert_pk_close(g, o, rtol=1e-5, noise=fl)
+    b.close()

@@ -5,6 +5,7 @@ import pytest
 from oracle import bk as Bo
 from oracle import cpu as C
 from pylians_amd import synth
+from tests._bkform import TRIANGLE_WAVES, closed_form_bk, plane_wave_theta
 
 pytestmark = pytest.mark.gpu
 
@@ -69,3 +70,53 @@ def test_bk_shells_past_nyquist(PKL, prec):
     assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= tol
     assert np.max(np.abs(g.B - o.B) / np.abs(o.B)) <= tol, (g.B, o.B)
     assert np.max(np.abs(g.Q - o.Q) / np.abs(o.Q)) <= tol
+
+
+@pytest.mark.parametrize("N,prec,ntheta", [(160, "f32", 3), (320, "f64", 2)])
+def test_bk_parity_row_pass_partial_wave_sizes(PKL, N, prec, ntheta):
+    """Grids whose c2r row pass has a thread count that is not a whole number of wavefronts before
+    rounding (80 points at N = 160, 160 at N = 320): every shell sum goes through that pass."""
+    L = 1000.0
+    pos = synth.zeldovich(N, L, seed=N + 1, rms_disp=5.0)
+    rho = C.deposit_f64(pos, N, L, "CIC")
+    delta = rho / rho.mean() - 1.0
+    delta = delta.astype(np.float32 if prec == "f32" else np.float64)
+    kF = 2 * np.pi / L
+    k1, k2 = 9.4 * kF, 17.2 * kF
+    theta = np.linspace(0.1, np.pi - 0.1, ntheta)
+    g = PKL.Bk(delta, L, k1, k2, theta, MAS="CIC")
+    o = Bo.Bk(delta.astype(np.float64), L, k1, k2, theta, MAS="CIC")
+    tol = 1e-5 if prec == "f32" else 1e-10
+    assert np.max(np.abs(g.Pk - o.Pk) / np.abs(o.Pk)) <= tol
+    assert np.max(np.abs(g.B - o.B) / np.abs(o.B)) <= tol, (g.B, o.B)
+    assert np.max(np.abs(g.Q - o.Q) / np.abs(o.Q)) <= tol
+
+
+@pytest.mark.parametrize("N", [8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640,
+                               768, 1024])
+def test_bk_closed_form_every_fft_size(PKL, N):
+    """Every grid size the FFT supports, float32, against a closed form (no FFT in the check): three
+    plane waves closing a triangle, n1 + n2 + n3 = 0, in shells k1, k2 and k3(theta); the shell sums
+    of the fused c2r row pass give Pk and B exactly up to float32 rounding."""
+    import torch
+    L = 2 * np.pi * 50.0                # kF = 0.02
+    kF = 2 * np.pi / L
+    waves = TRIANGLE_WAVES
+    k1, k2 = 3.0 * kF, 2.0 * kF
+    theta = plane_wave_theta()
+    x = torch.arange(N, device="cuda", dtype=torch.int64)
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    delta = torch.zeros((N, N, N), device="cuda", dtype=torch.float64)
+    for a, b, c in waves:
+        ph = torch.remainder(a * X + b * Y + c * Z, N).double()
+        delta += torch.cos(2 * np.pi * ph / N)
+        del ph
+    delta = delta.float()
+    g = PKL.Bk(delta, L, k1, k2, theta, MAS="None")
+    del delta
+    B, P = closed_form_bk(waves, N, L, g.k)
+    assert np.allclose(g.k[2], np.sqrt(5.0) * kF, rtol=1e-12)
+    scaleP = np.max(np.abs(P))
+    assert np.max(np.abs(g.Pk - P)) <= 1e-5 * scaleP, (g.Pk, P)
+    assert np.max(np.abs(g.B - B)) <= 1e-5 * np.max(np.abs(B)), (g.B, B)
+    torch.cuda.empty_cache()

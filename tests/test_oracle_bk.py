@@ -52,3 +52,17 @@ def test_empty_shell_raises():
     delta = np.random.default_rng(1).standard_normal((N, N, N))
     with pytest.raises(ZeroDivisionError):
         Bo.Bk(delta, 2 * np.pi, 100.0, 2.0, np.array([0.5]))
+
+
+@pytest.mark.parametrize("N", [8, 12, 16, 24])
+def test_oracle_matches_plane_wave_closed_form(N):
+    """The FP64 oracle and the plane-wave closed form (which the GPU tests use at every FFT size)
+    agree: both restate bispectrum_library.pyx:32-196, one through FFTs, one through lattice counts."""
+    from tests._bkform import TRIANGLE_WAVES, closed_form_bk, plane_wave_field, plane_wave_theta
+    L = 2 * np.pi * 50.0
+    kF = 2 * np.pi / L
+    o = Bo.Bk(plane_wave_field(N), L, 3 * kF, 2 * kF, plane_wave_theta(), MAS="None")
+    B, Pk = closed_form_bk(TRIANGLE_WAVES, N, L, o.k)
+    assert np.allclose(o.Pk, Pk, rtol=1e-12, atol=1e-12 * np.abs(Pk).max())
+    assert np.allclose(o.B, B, rtol=1e-12, atol=0)
+    assert np.all(B != 0) and np.all(Pk > 0)

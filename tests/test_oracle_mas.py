@@ -139,3 +139,20 @@ def test_c_oracle_matches_numpy_oracle():
         a = M.deposit(pos, 12, 7.0, s, W)
         b = C.deposit_f64(pos, 12, 7.0, s, W)
         assert np.allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("MAS", ["NGP", "CIC", "TSC", "PCS"])
+def test_parallel_oracle_deposit_is_bit_identical(MAS):
+    """The plane-split threaded oracle deposit (used at the benchmark sizes) gives the serial
+    loop's bits: every cell gets the same terms in the same order."""
+    from oracle import cpu as C
+    rng = np.random.default_rng(11)
+    n, dims, L = (1 << 20) + 4097, 40, 100.0
+    pos = (rng.random((n, 3)) * L).astype(np.float32)
+    pos[:7] = np.float32([0.0, L - 1e-4, L, 0.5 * L / dims, 99.99, 1e-7, 50.0])[:, None]
+    W = rng.random(n).astype(np.float32)
+    ser = C.deposit_f64(pos, dims, L, MAS, W, threads=1)
+    for t in (3, 8):
+        par = C.deposit_f64(pos, dims, L, MAS, W, threads=t)
+        assert np.array_equal(par, ser), (MAS, t)
+    assert ser.sum() == pytest.approx(W.astype(np.float64).sum(), rel=1e-12)

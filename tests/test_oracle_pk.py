@@ -134,7 +134,7 @@ def test_outputs_normalisation_and_keep_deltak():
     assert p.kpar.shape == p.kper.shape == p.Pk2D.shape
     assert p.kpar[0] == pytest.approx(0.5 * kF) and p.kper[N // 2 + 1] == pytest.approx(1.5 * kF)
     # keep_deltak: independent modes corrected, skipped modes left as in the FFT (Pk_library.pyx:327-355)
-    raw = np.fft.rfftn(delta)
+    raw = P.FFT3Dr(delta)
     _, _, _, indep = P.mode_grid(N)
     indep = np.broadcast_to(indep, raw.shape)
     assert np.array_equal(p.delta_k[~indep], raw[~indep])
