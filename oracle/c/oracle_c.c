@@ -190,3 +190,79 @@ void oracle_pkloop_f32ref(const float *dk, int dims, int axis, int p,
 {
     PKLOOP_BODY(float, float)
 }
+
+/* Linear-response variance of the mode-loop sums under white noise on the raw spectrum.
+ * A float32 pipeline returns delta_k + e_k with e_k white (E|e_k|^2 = s2).  To first order
+ * |C (delta_k + e_k)|^2 - |C delta_k|^2 = 2 C^2 Re(conj(delta_k) e_k), whose variance is
+ * 2 C^2 |C delta_k|^2 s2 (C: the window correction of the mode).  Per bin this accumulates
+ * sum 2 C^2 |C delta_k|^2 L_l(mu)^2 over the same independent modes and bins as the loop above
+ * (Pk_library.pyx:314-381): V1D (1-D), V2D (2-D), V3D (kmax+1, 3) for l = 0, 2, 4; the caller
+ * multiplies by s2.  The phase term ph = atan(re / r), r = |C delta|, moves by
+ * im (im dre - re dim) / (r (r^2 + re^2)) with (dre, dim) = C e, so ph^2 has variance
+ * 2 ph^2 C^2 s2 im^2 / (r^2 + re^2)^2: Vph (kmax+1).  Planes of kx are split among `threads` threads, each with its own
+ * histograms, summed at the end. */
+void oracle_pkloop_var(const double *dk, int dims, int axis, int p,
+                       double *V1D, double *V2D, double *V3D, double *Vph, int threads)
+{
+    int middle = dims / 2, even = (dims % 2 == 0), kmax_par = middle;
+    long nz = middle + 1;
+    long n1 = kmax_par + 1, n2 = (long)(kmax_par + 1) * (isqrt_l(2L * middle * middle) + 1);
+    long n3 = isqrt_l(3L * middle * middle) + 1;
+    double prefact = M_PI / dims;
+    int T = threads > 0 ? threads : 1, t;
+    size_t per = (size_t)(n1 + n2 + 4 * n3);
+    double *buf = (double *)calloc((size_t)T * per, sizeof(double));
+    if (!buf) return;
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (t = 0; t < T; t++) {
+        double *b1 = buf + (size_t)t * per, *b2 = b1 + n1, *b3 = b2 + n2, *bp = b3 + 3 * n3;
+        int kxx, kyy, kzz;
+        for (kxx = t; kxx < dims; kxx += T) {
+            int kx = kxx > middle ? kxx - dims : kxx;
+            double c0 = mas_corr(prefact * kx, p);
+            for (kyy = 0; kyy < dims; kyy++) {
+                int ky = kyy > middle ? kyy - dims : kyy;
+                double c1 = mas_corr(prefact * ky, p);
+                for (kzz = 0; kzz <= middle; kzz++) {
+                    int kz = kzz, k_par, k_per, k_index;
+                    double k, mu2, C, re, im, v, l2, l4, d2, ph;
+                    long o;
+                    if (kz == 0 || (kz == middle && even)) {
+                        if (kx < 0) continue;
+                        else if (kx == 0 || (kx == middle && even)) { if (ky < 0) continue; }
+                    }
+                    k = sqrt((double)(kx * kx + ky * ky + kz * kz));
+                    k_index = (int)isqrt_l((long)kx * kx + (long)ky * ky + (long)kz * kz);
+                    if (axis == 0) { k_par = kx; k_per = (int)isqrt_l((long)ky * ky + (long)kz * kz); }
+                    else if (axis == 1) { k_par = ky; k_per = (int)isqrt_l((long)kx * kx + (long)kz * kz); }
+                    else { k_par = kz; k_per = (int)isqrt_l((long)kx * kx + (long)ky * ky); }
+                    mu2 = (k == 0) ? 0.0 : (k_par / k) * (k_par / k);
+                    if (k_par < 0) k_par = -k_par;
+                    C = c0 * c1 * mas_corr(prefact * kz, p);
+                    o = ((long)kxx * dims + kyy) * nz + kzz;
+                    re = dk[2 * o] * C; im = dk[2 * o + 1] * C;
+                    d2 = re * re + im * im;
+                    v = 2.0 * C * C * d2;
+                    if (k <= middle) b1[k_par] += v;
+                    b2[(long)(kmax_par + 1) * k_per + k_par] += v;
+                    l2 = (3.0 * mu2 - 1.0) / 2.0;
+                    l4 = (35.0 * mu2 * mu2 - 30.0 * mu2 + 3.0) / 8.0;
+                    b3[3 * k_index + 0] += v;
+                    b3[3 * k_index + 1] += v * l2 * l2;
+                    b3[3 * k_index + 2] += v * l4 * l4;
+                    ph = atan2(re, sqrt(d2));
+                    if (d2 > 0.0) bp[k_index] += 2.0 * ph * ph * C * C * im * im / ((d2 + re * re) * (d2 + re * re));
+                }
+            }
+        }
+    }
+    for (t = 0; t < T; t++) {
+        const double *b1 = buf + (size_t)t * per, *b2 = b1 + n1, *b3 = b2 + n2, *bp = b3 + 3 * n3;
+        long i;
+        for (i = 0; i < n1; i++) V1D[i] += b1[i];
+        for (i = 0; i < n2; i++) V2D[i] += b2[i];
+        for (i = 0; i < 3 * n3; i++) V3D[i] += b3[i];
+        for (i = 0; i < n3; i++) Vph[i] += bp[i];
+    }
+    free(buf);
+}

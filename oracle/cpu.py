@@ -38,6 +38,8 @@ def lib():
                                                 ctypes.c_int]
         for f in (_lib.oracle_pkloop_f64, _lib.oracle_pkloop_f32ref):
             f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 9
+        _lib.oracle_pkloop_var.argtypes = ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                           + [ctypes.c_void_p] * 4 + [ctypes.c_int])
     return _lib
 
 
@@ -144,11 +146,13 @@ def cpu_reference_pipeline(pos, dims, BoxSize, axis, threads, MAS="CIC"):
     return res, t
 
 
-def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_floor=False, with_noise=False):
+def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_floor=False, with_noise=False,
+                         with_sigma=False):
     """FP64 oracle of the end-to-end path at full size (C deposit, float64 r2c, C float64 mode loop).
     With with_floor, also returns the float32 floor: the exact overdensity rounded once to float32,
     transformed with a float32 FFT (pocketfft single precision) and binned in float64.  With
-    with_noise (small grids), returns ``f32_noise_pk`` of the exact overdensity instead."""
+    with_noise (small grids), returns ``f32_noise_pk`` of the exact overdensity instead; with
+    with_sigma (any size), ``f32_sigma_pk``, its first-order closed form."""
     import scipy.fft
 
     rho = deposit_f64(pos, dims, BoxSize, MAS)
@@ -157,6 +161,13 @@ def fp64_oracle_pipeline(pos, dims, BoxSize, axis, MAS="CIC", workers=8, with_fl
     if with_noise:
         return (_pk.Pk(None, BoxSize, sums=pk_sums_c(np.fft.rfftn(rho), dims, axis, MAS), dims=dims),
                 f32_noise_pk(rho, BoxSize, axis, MAS))
+    if with_sigma:
+        d32 = rho.astype(np.float32)
+        dk = scipy.fft.rfftn(rho, workers=workers)
+        del rho
+        sums = pk_sums_c(dk, dims, axis, MAS)
+        o = _pk.Pk(None, BoxSize, sums=sums, dims=dims)
+        return o, f32_sigma_pk(dk, d32, BoxSize, axis, MAS, sums, workers=workers)
     d32 = rho.astype(np.float32) if with_floor else None
     dk = scipy.fft.rfftn(rho, workers=workers)
     del rho
@@ -207,6 +218,45 @@ def f32_noise_pk(delta, BoxSize, axis, MAS="CIC", samples=16, seed=0):
     for k in names:
         setattr(out, k, np.sqrt(acc[k] / samples))
     out.sigma_rel = float(np.sqrt(sigma2 / np.mean(np.abs(ex) ** 2)))
+    return out
+
+
+def f32_sigma_pk(dk, d32, BoxSize, axis, MAS, sums, workers=8):
+    """First-order closed form of ``f32_noise_pk``: per-bin standard deviation of the Pk outputs under
+    white noise e_k on the raw spectrum with the mean |e_k|^2 = sigma^2 that a float32 pipeline adds
+    (measured: ``d32``, the exact overdensity rounded once to float32, through pocketfft in single
+    precision, against ``dk``, the float64 spectrum of the exact field).  To first order a mode's
+    |C (delta_k + e_k)|^2 moves by 2 C^2 Re(conj(delta_k) e_k), of variance 2 C^2 |C delta_k|^2 sigma^2
+    (C: window correction), so each bin's variance is sigma^2 times a weighted mode sum
+    (liboracle ``oracle_pkloop_var``), and the normalisation of Pk_library.pyx:387-421 is linear in the
+    sums.  ``sums``: the float64 mode-loop sums of ``dk`` (for Nmodes and k)."""
+    import scipy.fft
+
+    N = dk.shape[0]
+    fl = scipy.fft.rfftn(d32, workers=workers)
+    sigma2 = 0.0
+    for i in range(0, N, 16):   # plane blocks: no full-size temporaries
+        sigma2 += float(np.sum(np.abs(fl[i:i + 16].astype(np.complex128) - dk[i:i + 16]) ** 2))
+    sigma2 /= dk.size
+    del fl
+    dkc = np.ascontiguousarray(dk, dtype=np.complex128)
+    _, _, kmax_par, kmax_per, kmax = _pk.frequencies(1.0, N)
+    V1 = np.zeros(kmax_par + 1)
+    V2 = np.zeros((kmax_par + 1) * (kmax_per + 1))
+    V3 = np.zeros((kmax + 1, 3))
+    Vp = np.zeros(kmax + 1)
+    lib().oracle_pkloop_var(_p(dkc), N, axis, _pk.MAS_function(MAS), _p(V1), _p(V2), _p(V3), _p(Vp),
+                            int(workers))
+    sd = dict(sums)
+    sd["Pk1D"] = np.sqrt(sigma2 * V1)
+    sd["Pk2D"] = np.sqrt(sigma2 * V2)
+    sd["P0"], sd["P2"], sd["P4"] = (np.sqrt(sigma2 * V3[:, i]) for i in range(3))
+    sd["Pkphase"] = np.sqrt(sigma2 * Vp)
+    X = _pk.Pk(None, BoxSize, sums=sd, dims=N)
+    out = _Std()
+    for k in ("Pk", "Pk1D", "Pk2D", "Pkphase"):
+        setattr(out, k, getattr(X, k))
+    out.sigma_rel = float(np.sqrt(sigma2 / np.mean(np.abs(dk) ** 2)))
     return out
 
 

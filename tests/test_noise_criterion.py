@@ -67,3 +67,27 @@ def test_noise_model_is_calibrated():
         if m.any():
             worst = max(worst, float(np.max(e[m] / (nz.Pk / P0)[m])))
     assert worst < 4.0, worst
+
+
+@pytest.mark.parametrize("MAS,axis", [("CIC", 0), ("PCS", 2)])
+def test_sigma_closed_form_matches_monte_carlo(MAS, axis):
+    """``f32_sigma_pk`` (first-order closed form, used at the benchmark sizes) against the Monte Carlo
+    spread of ``f32_noise_pk`` with many draws: the same per-bin standard deviations up to the
+    sampling error of the draws, and a closed form off by a factor sqrt(2) (a dropped factor 2 in
+    the variance) falls outside that band."""
+    pos = synth.zeldovich(32, 125.0, seed=9, rms_disp=0.6, sigma_v=40.0, axis=axis)
+    rho = C.deposit_f64(pos, 32, 125.0, MAS)
+    d = rho / (len(pos) / 32 ** 3) - 1.0
+    mc = C.f32_noise_pk(d, 125.0, axis, MAS, samples=96, seed=1)
+    dk = np.fft.rfftn(d)
+    sums = C.pk_sums_c(dk, 32, axis, MAS)
+    an = C.f32_sigma_pk(dk, d.astype(np.float32), 125.0, axis, MAS, sums, workers=4)
+    assert an.sigma_rel == pytest.approx(mc.sigma_rel, rel=1e-12)
+    for name in ("Pk", "Pk1D", "Pk2D", "Pkphase"):
+        a, m = getattr(an, name).ravel(), getattr(mc, name).ravel()
+        ok = (a > 0) & (m > 0)
+        r = m[ok] / a[ok]
+        # 96 draws: a per-bin sample standard deviation is within ~7 % (1 sigma) of the true one
+        assert 0.95 < np.median(r) < 1.05, (name, np.median(r))
+        assert np.mean(np.abs(r - 1.0) < 0.25) > 0.95, (name, np.mean(np.abs(r - 1.0) < 0.25))
+        assert not (0.95 < np.median(m[ok] / (np.sqrt(2.0) * a[ok])) < 1.05)
