@@ -49,6 +49,11 @@ extern "C" {
 #define PYL_BINNED 32    /* force the cell-binned LDS deposit */
 #define PYL_SORTED 64    /* force the cell-binned LDS deposit after a presort by supertile (for
                             particles in no spatial order) */
+#define PYL_ACC_F32W 1024  /* binned deposit into a float32 grid: 1-D weights and their products in
+                              float32 from the float64 cell fraction, float64 cell sums (cells within
+                              a few float32 ulps of the exact value; the same speed as the default) */
+#define PYL_ACC_F64 2048   /* binned deposit (the default): float64 weights and float64 cell sums, so
+                              every cell of a float32 grid is the correctly rounded exact value */
 
 /* Pk flags */
 #define PYL_PK_1D 1

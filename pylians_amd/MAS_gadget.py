@@ -1,18 +1,23 @@
-"""Density field of a Gadget snapshot on the GPU (MAS_gadget.py:7-83).
+"""Density field of a Gadget snapshot on the GPU, with the call surface of MAS_gadget.py:7-83.
 
 ``density_field_gadget(snapshot_fname, ptypes, dims, MAS='CIC', do_RSD=False, axis=0, verbose=True)``
-walks the files of the snapshot and the requested particle types, moves each chunk of positions to
-redshift space when asked, and accumulates it into one float32 grid that stays on the device
-until the end.  A single component is deposited unweighted when the mass table gives its mass;
-several components, or a component with a MASS block, are weighted by the particle masses in
-Msun/h.  Returns the (dims, dims, dims) float32 numpy grid.
+returns the (dims, dims, dims) float32 mass-assignment grid of the requested particle types
+(``[-1]``: all six) over every file of the snapshot, optionally in redshift space along `axis`.
+
+Design: the snapshot is staged once (``staging.Staged``: every file and type of one block end to end
+in a single device buffer), moved to redshift space in place on the device, and deposited by ONE
+call of the cell-binned deposit, with float64 per-cell accumulation and a single rounding.  The
+weighting follows the reference's rule: a single type whose mass the header's mass table gives is
+counted (no weights); otherwise every particle carries its mass in Msun/h (mass table or MASS
+block).  With `verbose`, the reference's progress lines are printed: the grid's total against the
+expected mass (or count), and the time taken.
 """
 import time
 
 import numpy as np
 import torch
 
-from . import _lib, readgadget
+from . import _lib, staging
 from .MAS_library import MA
 from .redshift_space_library import pos_redshift_space
 
@@ -20,45 +25,25 @@ __all__ = ["density_field_gadget"]
 
 
 def density_field_gadget(snapshot_fname, ptypes, dims, MAS="CIC", do_RSD=False, axis=0, verbose=True):
-    start = time.time()
+    t0 = time.time()
     if verbose:
         print("\nComputing density field of particles", ptypes)
-    density = torch.zeros((dims, dims, dims), dtype=torch.float32, device="cuda")
-    head = readgadget.header(snapshot_fname)
-    BoxSize = head.boxsize / 1e3          # Mpc/h
-    Masses = head.massarr * 1e10          # Msun/h
-    filenum = head.filenum
-    redshift = head.redshift
-    Hubble = head.Hubble
-    if ptypes == [-1]:
-        ptypes = [0, 1, 2, 3, 4, 5]
-    single_component = len(ptypes) == 1
-    num = 0.0
-    for i in range(filenum):
-        snapshot = snapshot_fname if filenum == 1 else "%s.%d" % (snapshot_fname, i)
-        npart = readgadget.header(snapshot).npart
-        for ptype in ptypes:
-            if npart[ptype] == 0:
-                continue
-            pos = _lib.to_dev(readgadget.read_field(snapshot, "POS ", ptype) / np.float32(1e3), torch.float32)
-            if do_RSD:
-                vel = _lib.to_dev(readgadget.read_field(snapshot, "VEL ", ptype), torch.float32)
-                pos_redshift_space(pos, vel, BoxSize, Hubble, redshift, axis)
-                del vel
-            if Masses[ptype] != 0:
-                if single_component:
-                    MA(pos, density, BoxSize, MAS)
-                    num += pos.shape[0]
-                else:
-                    mass = np.ones(int(npart[ptype]), dtype=np.float32) * np.float32(Masses[ptype])
-                    MA(pos, density, BoxSize, MAS, W=mass)
-                    num += np.sum(mass, dtype=np.float64)
-            else:
-                mass = readgadget.read_field(snapshot, "MASS", ptype) * np.float32(1e10)
-                MA(pos, density, BoxSize, MAS, W=mass)
-                num += np.sum(mass, dtype=np.float64)
-    out = _lib.to_host(density)
+    snap = staging.Staged(snapshot_fname, staging.expand_types(ptypes))
+    grid = torch.zeros((dims, dims, dims), dtype=torch.float32, device="cuda")
+    if snap.rows:
+        box = snap.box
+        pos = snap.positions()
+        if do_RSD:
+            pos_redshift_space(pos, snap.velocities(), box, snap.hubble_z, snap.head.redshift, axis)
+        counted = len(snap.types) == 1 and snap.mass_table[snap.types[0]] != 0
+        W = None if counted else snap.weights()
+        MA(pos, grid, box, MAS, W=W)
+        expected = float(snap.rows) if counted else float(W.double().sum())
+        del pos, W
+    else:
+        expected = 0.0
+    out = _lib.to_host(grid)
     if verbose:
-        print("%.8e should be equal to\n%.8e" % (np.sum(out, dtype=np.float64), num))
-        print("Time taken = %.2f seconds" % (time.time() - start))
+        print("%.8e should be equal to\n%.8e" % (np.sum(out, dtype=np.float64), expected))
+        print("Time taken = %.2f seconds" % (time.time() - t0))
     return out

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libpylians_hip.so")
 # constants of include/pylians_hip.h
 F32, F64 = 4, 8
 GRID_F64, DUP2D, RENORM2D, ZERO, ATOMIC, BINNED, SORTED = 1, 2, 4, 8, 16, 32, 64
+ACC_F32W, ACC_F64 = 1024, 2048
 PK_1D, PK_2D, PK_PHASE, PK_ALL, PK_ASYNC, PK_DROP_KZ0 = 1, 2, 4, 7, 256, 512
 E_ARG, E_UNSUPPORTED, E_MODES, E_ALLOC, E_LAUNCH = 1, 2, 3, 4, 5
 SCHEMES = {"NGP": 1, "CIC": 2, "TSC": 3, "PCS": 4}

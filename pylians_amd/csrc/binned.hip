@@ -77,6 +77,52 @@ template <> struct Axis<4> {
     }
 };
 
+// The same stencils with the 1-D weights evaluated in float32 from the float64 cell fraction
+// (CIC: u and 1 - u in float64, each rounded once; TSC/PCS: the distance |cell - d| and its
+// distance to the stencil's outer edge in float64, each rounded once, then the polynomials in
+// float32, so that no weight loses precision to cancellation); the lowest cell is the float64 one.
+template <int S> struct Axis32;
+template <> struct Axis32<1> {
+    static __device__ __forceinline__ double eval(double d, float* w) {
+        w[0] = 1.0f;
+        return floor(d + 0.5);
+    }
+};
+template <> struct Axis32<2> {
+    static __device__ __forceinline__ double eval(double d, float* w) {
+        const double b = floor(d);
+        const double u = d - b;
+        w[0] = (float)(1.0 - u);
+        w[1] = (float)u;
+        return b;
+    }
+};
+template <> struct Axis32<3> {
+    static __device__ __forceinline__ double eval(double d, float* w) {
+        const double mn = floor(d - 1.5);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double s = fabs(mn + (j + 1) - d);
+            const float sf = (float)s, t = (float)(1.5 - s);   // distance to the outer edge, exact first
+            w[j] = s < 0.5 ? 0.75f - sf * sf : (s < 1.5 ? 0.5f * t * t : 0.0f);
+        }
+        return mn + 1.0;
+    }
+};
+template <> struct Axis32<4> {
+    static __device__ __forceinline__ double eval(double d, float* w) {
+        const double mn = floor(d - 2.0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double s = fabs(mn + (j + 1) - d);
+            const float sf = (float)s, t = (float)(2.0 - s);
+            w[j] = s < 1.0 ? (4.0f - 6.0f * sf * sf + 3.0f * sf * sf * sf) * (1.0f / 6.0f)
+                           : (s < 2.0 ? t * t * t * (1.0f / 6.0f) : 0.0f);
+        }
+        return mn + 1.0;
+    }
+};
+
 // periodic cell index of an integral float64 cell coordinate b (positions inside the box give
 // 0 <= b < N directly, a stencil edge at most one box away; others reduce with a float64 floor)
 __device__ __forceinline__ int wrapN(double b, int N) {
@@ -352,7 +398,46 @@ __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, i
     }
 }
 
-template <int S, typename G, bool HASW, int MODE, bool UNIT>
+// accumulate() with the 1-D weights and their products in float32 (Axis32); each contribution is
+// added to the float64 LDS cell
+template <int S, bool HASW, bool CLIP, bool UNIT>
+__device__ __forceinline__ void accumulate32(double* acc, const float* r, int N, int ox, int oy, int oz, double inv,
+                                             float mult) {
+    float w0[S], w1[S], w2[S];
+    int l0 = wrapN(Axis32<S>::eval((double)r[0] * inv, w0), N) - ox;
+    int l1 = wrapN(Axis32<S>::eval((double)r[1] * inv, w1), N) - oy;
+    int l2 = wrapN(Axis32<S>::eval((double)r[2] * inv, w2), N) - oz;
+    if (l0 >= TILE) l0 -= N;
+    if (l1 >= TILE) l1 -= N;
+    if (l2 >= TILE) l2 -= N;
+    const float ww = HASW ? r[3] * mult : mult;
+#pragma unroll
+    for (int l = 0; l < S; ++l) {
+        const int c0 = l0 + l;
+        if (CLIP && (c0 < 0 || c0 >= TILE)) continue;
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+            const int c1 = l1 + m;
+            if (CLIP && (c1 < 0 || c1 >= TILE)) continue;
+            const float wlm = UNIT ? w0[l] * w1[m] : w0[l] * w1[m] * ww;
+            double* row = acc + (c0 * TILE + c1) * TILE;
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                const int c2 = l2 + q;
+                if (!CLIP || (c2 >= 0 && c2 < TILE)) atomicAdd(row + c2, (double)(wlm * w2[q]));
+            }
+        }
+    }
+}
+
+// ACC: ACC_F64 (float64 weights, float64 cell sums) or ACC_F32W (float32 weights and products,
+// float64 cell sums).  Both run at the same speed on gfx950 (the kernel is bound by loads and LDS
+// atomics, not by float64 arithmetic); a float32 LDS cell (ds_add_f32) was measured about six times
+// slower than the float64 one (ds_add_f64) in this kernel, and a TwoSum-compensated float32 cell
+// (returning atomics) twelve times slower, so the float64 cell is the accumulator of every mode.
+enum { ACC_F64 = 0, ACC_F32W = 1 };
+
+template <int S, typename G, bool HASW, int MODE, bool UNIT, int ACC>
 __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     constexpr int R = HASW ? 4 : 3;
     constexpr int NE = 72;   // 8 neighbours (d = 0 own) x (8 class segments + 1 overflow group)
@@ -452,7 +537,12 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
             // an overflow group's record reaches this tile when its class contains the offset
             const unsigned d = (unsigned)(ej / 9);
             const bool ok = ej >= 0 && (ej % 9 < 8 || (cg[j] & d) == d);
-            if (ej == 0)
+            if (ACC == ACC_F32W) {
+                if (ej == 0)
+                    accumulate32<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, (float)a.mult);
+                else if (ok)
+                    accumulate32<S, HASW, true, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, (float)a.mult);
+            } else if (ej == 0)
                 accumulate<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
             else if (ok)
                 accumulate<S, HASW, true, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
@@ -601,7 +691,7 @@ bool binned_supported(int axes, int dims) { return axes == 3 && dims % TILE == 0
 
 template <int S, typename G, bool HASW, int MODE>
 static int run_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
-                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd) {
+                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd, int acc) {
     constexpr int R = HASW ? 4 : 3;
     int rc = OK;
     const int nt = dims / TILE;
@@ -666,10 +756,14 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
             ta.cap[c] = cap[c];
         }
         const dim3 tg((unsigned)((ntiles + XCDS - 1) / XCDS * XCDS));
-        if (!HASW && mult == 1.0)
-            hipLaunchKernelGGL((k_tile<S, G, HASW, MODE, true>), tg, dim3(TT), 0, st, ta);
-        else
-            hipLaunchKernelGGL((k_tile<S, G, HASW, MODE, false>), tg, dim3(TT), 0, st, ta);
+        const bool unit = !HASW && mult == 1.0;
+#define PYL_TILE_LAUNCH(U, A) hipLaunchKernelGGL((k_tile<S, G, HASW, MODE, U, A>), tg, dim3(TT), 0, st, ta)
+        if (acc == ACC_F32W) {
+            if (unit) PYL_TILE_LAUNCH(true, ACC_F32W); else PYL_TILE_LAUNCH(false, ACC_F32W);
+        } else {
+            if (unit) PYL_TILE_LAUNCH(true, ACC_F64); else PYL_TILE_LAUNCH(false, ACC_F64);
+        }
+#undef PYL_TILE_LAUNCH
         PYL_HIP(hipGetLastError());
     }
     return OK;
@@ -677,19 +771,19 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
 
 template <int S, typename G, int MODE>
 static int dispatch_w(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
-                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd) {
-    return W ? run_binned<S, G, true, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd)
-             : run_binned<S, G, false, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
+                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd, int acc) {
+    return W ? run_binned<S, G, true, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd, acc)
+             : run_binned<S, G, false, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd, acc);
 }
 
 template <typename G, int MODE>
 static int dispatch_s(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, double mult,
-                      void* grid, const double* dmass, double mass, hipStream_t st, const Rsd& rsd) {
+                      void* grid, const double* dmass, double mass, hipStream_t st, const Rsd& rsd, int acc) {
     switch (scheme) {
-        case 1: return dispatch_w<1, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
-        case 2: return dispatch_w<2, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
-        case 3: return dispatch_w<3, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
-        case 4: return dispatch_w<4, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd);
+        case 1: return dispatch_w<1, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd, acc);
+        case 2: return dispatch_w<2, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd, acc);
+        case 3: return dispatch_w<3, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd, acc);
+        case 4: return dispatch_w<4, G, MODE>(pos, W, n, dims, BoxSize, mult, grid, dmass, mass, st, rsd, acc);
         default: set_error("unknown scheme %d", scheme); return E_ARG;
     }
 }
@@ -741,10 +835,11 @@ static int presort(const float* pos, const float* W, int64_t n, int dims, double
     return OK;
 }
 
-// mode: 0 accumulate, 1 overwrite, 2 overdensity (mass = dmass[0] if dmass else mass)
+// mode: 0 accumulate, 1 overwrite, 2 overdensity (mass = dmass[0] if dmass else mass); acc: cell
+// accumulation of a float32 grid (ACC_F64 / ACC_F32W; float64 grids always use ACC_F64)
 int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
                    int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st,
-                   const Rsd& rsd_in, bool sorted) {
+                   const Rsd& rsd_in, bool sorted, int acc) {
     PYL_CHECK(!rsd_in.vel || (rsd_in.axis >= 0 && rsd_in.axis <= 2), E_ARG, "redshift-space axis must be 0, 1 or 2");
     PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs 3-D grids with dims %% %d == 0", TILE);
     PYL_CHECK(n > 0, E_ARG, "binned deposit needs particles");
@@ -759,13 +854,13 @@ int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double
         rsd = Rsd();
     }
     if (f64) {
-        if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
-        if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
-        return dispatch_s<double, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
+        if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, ACC_F64);
+        if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, ACC_F64);
+        return dispatch_s<double, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, ACC_F64);
     }
-    if (mode == OUT_ACC) return dispatch_s<float, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
-    if (mode == OUT_SET) return dispatch_s<float, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
-    return dispatch_s<float, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd);
+    if (mode == OUT_ACC) return dispatch_s<float, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
+    if (mode == OUT_SET) return dispatch_s<float, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
+    return dispatch_s<float, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
 }
 
 }  // namespace pyl

@@ -20,7 +20,7 @@ using namespace pyl;
 
 static_assert(PYL_GRID_F64 == DEP_GRID_F64 && PYL_DUP2D == DEP_DUP2D && PYL_RENORM2D == DEP_RENORM2D &&
                   PYL_ZERO == DEP_ZERO && PYL_ATOMIC == DEP_ATOMIC && PYL_BINNED == DEP_BINNED &&
-                  PYL_SORTED == DEP_SORTED,
+                  PYL_SORTED == DEP_SORTED && PYL_ACC_F32W == DEP_ACC_F32W && PYL_ACC_F64 == DEP_ACC_F64,
               "flag mismatch");
 static_assert(PYL_PK_1D == PK_1D && PYL_PK_2D == PK_2D && PYL_PK_PHASE == PK_PHASE && PYL_PK_ASYNC == PK_ASYNC &&
                   PYL_PK_DROP_KZ0 == PK_DROP_KZ0,
@@ -169,9 +169,10 @@ static int check_modes(const PkGeometry* g, const PkSums& sums, int flags, hipSt
     return E_MODES;
 }
 
+namespace pyl {
 template <typename R>
-static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int mas_p, int flags, double* out,
-                         cplx<R>* keep, bool affine, const double* dmass, double mass, hipStream_t st) {
+int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int mas_p, int flags, double* out,
+                  cplx<R>* keep, bool affine, const double* dmass, double mass, hipStream_t st) {
     int rc = OK;
     // rows of the internal spectrum padded to 128 bytes: the strided column passes then read whole
     // cache lines (kz = N/2+1 .. P-1 are never touched)
@@ -207,6 +208,11 @@ static int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int
     }
     return check_modes(g, sums, flags, st);
 }
+template int pk_from_delta<float>(const float*, int, double, int, int, int, double*, cplx<float>*, bool, const double*,
+                                  double, hipStream_t);
+template int pk_from_delta<double>(const double*, int, double, int, int, int, double*, cplx<double>*, bool,
+                                   const double*, double, hipStream_t);
+}  // namespace pyl
 
 extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags,
                       double* out, void* deltak_out, void* stream) {
@@ -247,10 +253,11 @@ bool g_tune_on = true;
 // (pass Z then applies delta = rho * a - 1 in float64).  The float32 atomic path accumulates in a
 // float64 grid and rounds delta once, like the binned path, so both give the same float32 field.
 static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                         int kind, void* grid, const double* dmass, hipStream_t st, const Rsd& rsd = Rsd()) {
+                         int kind, void* grid, const double* dmass, hipStream_t st, const Rsd& rsd = Rsd(),
+                         int flags = 0) {
     if (kind != DK_ATOMIC)
         return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, 2, 1.0, grid, dmass, (double)n, st, rsd,
-                              kind == DK_SORTED);
+                              kind == DK_SORTED, acc_mode(flags));
     if (f64) return deposit(pos, W, n, 3, dims, BoxSize, scheme, DEP_ZERO | DEP_ATOMIC | DEP_GRID_F64, grid, st, rsd);
     const int64_t cells = (int64_t)dims * dims * dims;
     int rc = OK;
@@ -262,7 +269,8 @@ static int deposit_delta(const float* pos, const float* W, int64_t n, int dims, 
 }
 
 static int tune_choice(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
-                       void* grid, const double* dmass, hipStream_t st, int* kind, const Rsd& rsd = Rsd()) {
+                       void* grid, const double* dmass, hipStream_t st, int* kind, const Rsd& rsd = Rsd(),
+                       int flags = 0) {
     if (!binned_supported(3, dims)) {
         *kind = DK_ATOMIC;
         return OK;
@@ -271,7 +279,7 @@ static int tune_choice(const float* pos, const float* W, int64_t n, int dims, do
     PYL_HIP(hipGetDevice(&dev));
     int nlog = 0;
     while ((1LL << (nlog + 1)) <= n) ++nlog;
-    TuneKey key{dev, dims, scheme, f64 ? 8 : 4, W ? 1 : 0, nlog};
+    TuneKey key{dev, dims, scheme, f64 ? 8 : (4 + acc_mode(flags) * 16), W ? 1 : 0, nlog};
     {
         std::lock_guard<std::mutex> lk(g_tmu);
         auto it = g_tune.find(key);
@@ -295,11 +303,11 @@ static int tune_choice(const float* pos, const float* W, int64_t n, int dims, do
     float ms[3] = {0.f, 0.f, 0.f};
     int rc = OK;
     for (int c = 0; c < 3 && rc == OK; ++c) {
-        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd);  // warm
+        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);  // warm
         if (rc) break;
         PYL_HIP(hipEventRecord(e[0], st));
         for (int r = 0; r < 2 && rc == OK; ++r)
-            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd);
+            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);
         PYL_HIP(hipEventRecord(e[1], st));
         PYL_HIP(hipEventSynchronize(e[1]));
         PYL_HIP(hipEventElapsedTime(&ms[c], e[0], e[1]));
@@ -375,10 +383,10 @@ static int pk_particles(const float* pos, const float* W, int64_t n, int dims, d
         PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
         kind = (flags & DEP_SORTED) ? DK_SORTED : DK_BINNED;
     } else {
-        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &kind, rsd);
+        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &kind, rsd, flags);
         if (rc) return rc;
     }
-    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, kind, grid, dmass, st, rsd);
+    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, kind, grid, dmass, st, rsd, flags);
     if (rc) return rc;
     const int pkflags = flags & (PYL_PK_ALL | PYL_PK_ASYNC | PYL_PK_DROP_KZ0);
     if (f64)

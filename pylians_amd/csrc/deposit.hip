@@ -181,7 +181,7 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
     for (int a = 0; a < axes; ++a) cells *= dims;
     if (n > 0 && !(flags & DEP_ATOMIC) && binned_supported(axes, dims))
         return deposit_binned(pos, W, n, dims, BoxSize, scheme, f64, (flags & DEP_ZERO) ? 1 : 0, 1.0, grid, nullptr,
-                              0.0, st, rsd, (flags & DEP_SORTED) != 0);
+                              0.0, st, rsd, (flags & DEP_SORTED) != 0, acc_mode(flags));
     PYL_CHECK(!(flags & (DEP_BINNED | DEP_SORTED)), E_UNSUPPORTED, "binned deposit needs a 3-D grid with dims %% 16 == 0");
     if (flags & DEP_ZERO) {
         StageTimer t("grid_zero", st);

@@ -12,7 +12,13 @@ enum DepositFlags {
     DEP_ATOMIC = 16,    // force the direct global-atomic kernel
     DEP_BINNED = 32,    // force the cell-binned LDS kernel
     DEP_SORTED = 64,    // force the cell-binned LDS kernel after a presort by supertile
+    DEP_ACC_F32W = 1024,   // binned deposit into a float32 grid: float32 weights, float64 cell sums
+    DEP_ACC_F64 = 2048,    // binned deposit: float64 weights and cell sums (every cell correctly rounded)
 };
+
+// cell accumulation of the binned deposit into a float32 grid: 0 float64 weights and sums (default),
+// 1 float32 weights and float64 sums (the ACC_* of binned.hip)
+inline int acc_mode(int flags) { return (flags & DEP_ACC_F32W) && !(flags & DEP_ACC_F64) ? 1 : 0; }
 
 // Redshift-space shift fused into the position load (redshift_space_library.pyx:29-43):
 // x[axis] <- x[axis] + vel[axis] * fac in float32, fac = (float)((1 + z) / H), and
@@ -52,6 +58,6 @@ int delta_round(const double* rho, float* out, int64_t n, const double* dmass, d
 bool binned_supported(int axes, int dims);
 int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
                    int mode, double mult, void* grid, const double* dmass, double mass, hipStream_t st,
-                   const Rsd& rsd = Rsd(), bool sorted = false);
+                   const Rsd& rsd = Rsd(), bool sorted = false, int acc = 0);
 
 }  // namespace pyl

@@ -318,7 +318,30 @@ int cu_count() {
     return n;
 }
 
+// host table exp(-2 pi i t / N), t < N, rounded once from long double; cached per N
+template <typename R> static const std::vector<cplx<R>>& host_twiddles(int N) {
+    static std::mutex mu;
+    static std::map<int, std::vector<cplx<R>>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(N);
+    if (it != cache.end()) return it->second;
+    std::vector<cplx<R>>& h = cache[N];
+    h.resize(N);
+    for (int t = 0; t < N; ++t) {
+        long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)N;
+        h[t].x = (R)cosl(ang);
+        h[t].y = (R)sinl(ang);
+    }
+    return h;
+}
+
+// device copy: cached per (device, N); inside a fixed arena a fresh copy in the arena
 template <typename R> const cplx<R>* twiddles(int N) {
+    const std::vector<cplx<R>>& h = host_twiddles<R>(N);
+    if (arena_fixed()) {
+        int rc = OK;
+        return (const cplx<R>*)arena_upload(h.data(), sizeof(cplx<R>) * N, &rc);
+    }
     static std::mutex mu;
     static std::map<std::pair<int, int>, void*> cache;
     int dev = 0;
@@ -327,12 +350,6 @@ template <typename R> const cplx<R>* twiddles(int N) {
     auto key = std::make_pair(dev, N);
     auto it = cache.find(key);
     if (it != cache.end()) return (const cplx<R>*)it->second;
-    std::vector<cplx<R>> h(N);
-    for (int t = 0; t < N; ++t) {
-        long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)N;
-        h[t].x = (R)cosl(ang);
-        h[t].y = (R)sinl(ang);
-    }
     void* d = nullptr;
     if (hipMalloc(&d, sizeof(cplx<R>) * N) != hipSuccess) return nullptr;
     if (hipMemcpy(d, h.data(), sizeof(cplx<R>) * N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;

@@ -94,11 +94,73 @@ static void geom_slab(int N, int axis, const PkDims& d, int x0, int x1, std::vec
     }
 }
 
+// host half of the geometry (device pointers null), cached per (N, axis)
+static const PkGeometry* host_geometry(int N, int axis) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, PkGeometry*> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(N, axis);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    PkGeometry* g = new PkGeometry();
+    g->d = pk_dims(N, half_len(N));
+    g->axis = axis;
+    g->n1 = g->n2 = g->n3 = nullptr;
+    g->k3sum = nullptr;
+    const PkDims& d = g->d;
+    g->h_n1.assign(d.n1(), 0);
+    g->h_n2.assign(d.n2(), 0);
+    g->h_n3.assign(d.n3(), 0);
+    g->h_k3sum.assign(d.n3(), 0.0);
+    unsigned nth = std::thread::hardware_concurrency();
+    nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
+    if ((int)nth > N) nth = N;
+    std::vector<std::vector<unsigned long long>> a1(nth, std::vector<unsigned long long>(d.n1(), 0)),
+        a2(nth, std::vector<unsigned long long>(d.n2(), 0)), a3(nth, std::vector<unsigned long long>(d.n3(), 0));
+    std::vector<std::vector<double>> ak(nth, std::vector<double>(d.n3(), 0.0));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; ++t) {
+        const int x0 = (int)((long long)N * t / nth), x1 = (int)((long long)N * (t + 1) / nth);
+        pool.emplace_back(geom_slab, N, axis, std::cref(d), x0, x1, std::ref(a1[t]), std::ref(a2[t]),
+                          std::ref(a3[t]), std::ref(ak[t]));
+    }
+    for (auto& th : pool) th.join();
+    for (unsigned t = 0; t < nth; ++t) {
+        for (int i = 0; i < d.n1(); ++i) g->h_n1[i] += a1[t][i];
+        for (int i = 0; i < d.n2(); ++i) g->h_n2[i] += a2[t][i];
+        for (int i = 0; i < d.n3(); ++i) {
+            g->h_n3[i] += a3[t][i];
+            g->h_k3sum[i] += ak[t][i];
+        }
+    }
+    cache[key] = g;
+    return g;
+}
+
+// Device geometry: cached per (device, N, axis); inside a fixed arena, a per-thread copy whose
+// device arrays are fresh uploads into the arena.
 const PkGeometry* pk_geometry(int N, int axis, int* rc) {
+    *rc = OK;
+    const PkGeometry* hg = host_geometry(N, axis);
+    const PkDims& d = hg->d;
+    const size_t b1 = sizeof(unsigned long long) * d.n1(), b2 = sizeof(unsigned long long) * d.n2(),
+                 b3 = sizeof(unsigned long long) * d.n3(), bk = sizeof(double) * d.n3();
+    if (arena_fixed()) {
+        static thread_local PkGeometry tg;
+        tg = *hg;
+        tg.n1 = (unsigned long long*)arena_upload(hg->h_n1.data(), b1, rc);
+        if (*rc) return nullptr;
+        tg.n2 = (unsigned long long*)arena_upload(hg->h_n2.data(), b2, rc);
+        if (*rc) return nullptr;
+        tg.n3 = (unsigned long long*)arena_upload(hg->h_n3.data(), b3, rc);
+        if (*rc) return nullptr;
+        tg.k3sum = (double*)arena_upload(hg->h_k3sum.data(), bk, rc);
+        if (*rc) return nullptr;
+        return &tg;
+    }
     static std::mutex mu;
     static std::map<std::tuple<int, int, int>, PkGeometry*> cache;
     int dev = 0;
-    *rc = OK;
     if (hipGetDevice(&dev) != hipSuccess) {
         set_error("hipGetDevice failed");
         *rc = E_LAUNCH;
@@ -108,12 +170,7 @@ const PkGeometry* pk_geometry(int N, int axis, int* rc) {
     auto key = std::make_tuple(dev, N, axis);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    PkGeometry* g = new PkGeometry();
-    g->d = pk_dims(N, half_len(N));
-    g->axis = axis;
-    const PkDims& d = g->d;
-    size_t b1 = sizeof(unsigned long long) * d.n1(), b2 = sizeof(unsigned long long) * d.n2(),
-           b3 = sizeof(unsigned long long) * d.n3(), bk = sizeof(double) * d.n3();
+    PkGeometry* g = new PkGeometry(*hg);
     char* base = nullptr;
     if (hipMalloc((void**)&base, b1 + b2 + b3 + bk) != hipSuccess) {
         set_error("geometry allocation failed");
@@ -125,33 +182,6 @@ const PkGeometry* pk_geometry(int N, int axis, int* rc) {
     g->n1 = (unsigned long long*)(base + bk);
     g->n2 = (unsigned long long*)(base + bk + b1);
     g->n3 = (unsigned long long*)(base + bk + b1 + b2);
-    g->h_n1.assign(d.n1(), 0);
-    g->h_n2.assign(d.n2(), 0);
-    g->h_n3.assign(d.n3(), 0);
-    g->h_k3sum.assign(d.n3(), 0.0);
-    {
-        unsigned nth = std::thread::hardware_concurrency();
-        nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
-        if ((int)nth > N) nth = N;
-        std::vector<std::vector<unsigned long long>> a1(nth, std::vector<unsigned long long>(d.n1(), 0)),
-            a2(nth, std::vector<unsigned long long>(d.n2(), 0)), a3(nth, std::vector<unsigned long long>(d.n3(), 0));
-        std::vector<std::vector<double>> ak(nth, std::vector<double>(d.n3(), 0.0));
-        std::vector<std::thread> pool;
-        for (unsigned t = 0; t < nth; ++t) {
-            const int x0 = (int)((long long)N * t / nth), x1 = (int)((long long)N * (t + 1) / nth);
-            pool.emplace_back(geom_slab, N, axis, std::cref(d), x0, x1, std::ref(a1[t]), std::ref(a2[t]),
-                              std::ref(a3[t]), std::ref(ak[t]));
-        }
-        for (auto& th : pool) th.join();
-        for (unsigned t = 0; t < nth; ++t) {
-            for (int i = 0; i < d.n1(); ++i) g->h_n1[i] += a1[t][i];
-            for (int i = 0; i < d.n2(); ++i) g->h_n2[i] += a2[t][i];
-            for (int i = 0; i < d.n3(); ++i) {
-                g->h_n3[i] += a3[t][i];
-                g->h_k3sum[i] += ak[t][i];
-            }
-        }
-    }
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMemcpy(g->n1, g->h_n1.data(), b1, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(g->n2, g->h_n2.data(), b2, hipMemcpyHostToDevice);
@@ -170,7 +200,30 @@ const PkGeometry* pk_geometry(int N, int axis, int* rc) {
 
 // ------------------------------------------------------------------ window tables
 // win[i] = (x/sin x)^p with x = pi kfreq(i)/N (1 at x = 0): Pk_library.pyx:86-87, 316-324
+static const std::vector<double>& host_window(int N, int p) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(N, p);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double>& h = cache[key];
+    h.resize(N);
+    for (int i = 0; i < N; ++i) {
+        int k = i > N / 2 ? i - N : i;
+        double x = M_PI / N * k;
+        h[i] = (x == 0.0) ? 1.0 : pow(x / sin(x), p);
+    }
+    return h;
+}
+
+// device copy cached per (device, N, p); inside a fixed arena a fresh copy in the arena
 static const double* window_table(int N, int p) {
+    const std::vector<double>& h = host_window(N, p);
+    if (arena_fixed()) {
+        int rc = OK;
+        return (const double*)arena_upload(h.data(), sizeof(double) * N, &rc);
+    }
     static std::mutex mu;
     static std::map<std::tuple<int, int, int>, double*> cache;
     int dev = 0;
@@ -179,12 +232,6 @@ static const double* window_table(int N, int p) {
     auto key = std::make_tuple(dev, N, p);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    std::vector<double> h(N);
-    for (int i = 0; i < N; ++i) {
-        int k = i > N / 2 ? i - N : i;
-        double x = M_PI / N * k;
-        h[i] = (x == 0.0) ? 1.0 : pow(x / sin(x), p);
-    }
     double* dptr = nullptr;
     if (hipMalloc((void**)&dptr, sizeof(double) * N) != hipSuccess) return nullptr;
     if (hipMemcpy(dptr, h.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
@@ -199,11 +246,11 @@ constexpr int isqrt_c(long long v) {
     while ((r + 1) * (r + 1) <= v) ++r;
     return (int)r;
 }
-// LDS bytes of the histograms: 3-D (4 float64 sums + a mode count), 1-D and the widest 2-D
-// window (NL k_par rows), each a float64 sum and a 32-bit mode count
-constexpr int hist_bytes(int N, int NL) {
+// LDS bytes of the histograms: 3-D (4 float64 sums + a mode count), 1-D and the 2-D window (one
+// k_par row; NL rows when blocks own a kz block, fixk = 1), each a float64 sum and a 32-bit count
+constexpr int hist_bytes(int N, int NL, bool fixk) {
     return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * (4 * 8 + 4) + (N / 2 + 1) * 12 +
-           NL * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 12;
+           (fixk ? NL : 1) * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 12;
 }
 // lines per block for the binning kernel: at most 8, and few enough that two blocks share a CU
 // (160 KB of LDS) beside their histograms
@@ -211,10 +258,10 @@ constexpr int hist_bytes(int N, int NL) {
 constexpr int threads_bin_c(int N, int NL) {
     return clamp_i(N * NL / 16 * (N % 3 == 0 && N >= 192 ? 2 : 1), 64, 512);
 }
-template <typename R, int N> constexpr int nl_bin() {
+template <typename R, int N, bool FIXK> constexpr int nl_bin() {
     int nl = 8;
     while (nl > 2 && (N + threads_bin_c(N, nl) / nl) * nl * (int)sizeof(cplx<R>) +
-                             N * (int)(sizeof(cplx<R>) + sizeof(R)) + hist_bytes(N, nl) > 80 * 1024)
+                             N * (int)(sizeof(cplx<R>) + sizeof(R)) + hist_bytes(N, nl, FIXK) > 80 * 1024)
         nl /= 2;
     return nl;
 }
@@ -606,10 +653,10 @@ static int num_cus() {
 
 int pk_last_axis(int axis) { return axis == 0 ? 1 : 0; }
 
-template <typename R, int N>
-static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
-                           const PkSums& sums, hipStream_t st) {
-    constexpr int NL = nl_bin<R, N>();
+template <typename R, int N, bool FIXK>
+static int launch_last_bin_k(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
+                             const PkSums& sums, hipStream_t st) {
+    constexpr int NL = nl_bin<R, N, FIXK>();
     constexpr int T = threads_bin<N, NL>();
     const int W = N / 2 + 1;   // kz = 0 .. N/2; rows of the spectrum may be padded to P >= W
     const PkDims d = pk_dims(N, W);
@@ -617,7 +664,7 @@ static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int ax
     const double* win = window_table(N, p);
     PYL_CHECK(tw && win, E_ALLOC, "table allocation failed");
     const int last = pk_last_axis(axis);
-    const int fixk = axis == 2 ? 1 : 0;
+    const int fixk = FIXK ? 1 : 0;
     const int nkzb = (W + NL - 1) / NL;
     // about two blocks per CU, all resident at once, each walking its column blocks with prefetch;
     // for the line of sight along z (kz blocks split over u chunks) four per CU measured faster
@@ -651,6 +698,14 @@ static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int ax
                        nper, d.kmax_par, fixk, nchunk, NL, N, sums.s2, sums.c2);
     PYL_HIP(hipGetLastError());
     return OK;
+}
+
+// blocks own one kz block (fixk) when the line of sight is z, else one u: separate LDS budgets
+template <typename R, int N>
+static int launch_last_bin(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
+                           const PkSums& sums, hipStream_t st) {
+    return axis == 2 ? launch_last_bin_k<R, N, true>(spec, keep, P, axis, p, flags, sums, st)
+                     : launch_last_bin_k<R, N, false>(spec, keep, P, axis, p, flags, sums, st);
 }
 
 #define PYL_PK_SIZES(X) \

@@ -67,4 +67,12 @@ int pk_last_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int a
 size_t pk_out_len(const PkDims& d);
 int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out, hipStream_t st);
 
+// The whole estimator from a real (N,N,N) field: pass Z r2c (affine: delta = rho * N^3 / mass - 1
+// applied on the load, mass = dmass[0] when dmass is non-null), the column pass, the fused last pass
+// + binning, the normalisation and the mode check (see pyl_pk).  keep (nullable) receives the
+// corrected spectrum.
+template <typename R>
+int pk_from_delta(const R* delta, int dims, double BoxSize, int axis, int mas_p, int flags, double* out,
+                  cplx<R>* keep, bool affine, const double* dmass, double mass, hipStream_t st);
+
 }  // namespace pyl

@@ -69,8 +69,24 @@ struct WsArena {
     int dev = 0;
     bool sealed = false;
     std::map<int, std::pair<void*, size_t>> slots;
+    // fixed arena: every slot and table is carved from [base, base + cap) in request order
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    hipStream_t st = nullptr;
 };
 static thread_local WsArena* t_arena = nullptr;
+
+static void* carve(WsArena* a, size_t bytes, int* rc) {
+    const size_t at = (a->used + 255) / 256 * 256;
+    if (at > a->cap || bytes > a->cap - at) {
+        set_error("scratch of %zu bytes exhausted: this call needs more than %zu bytes (a larger grid than the "
+                  "no-allocation entry point supports)", a->cap, at + bytes);
+        *rc = E_ALLOC;
+        return nullptr;
+    }
+    a->used = at + bytes;
+    return a->base + at;
+}
 
 static void* grow(std::pair<void*, size_t>& e, size_t bytes, int* rc) {
     if (e.first) {
@@ -108,6 +124,11 @@ void* workspace(int slot, size_t bytes, int* rc) {
         }
         auto& e = a->slots[slot];
         if (e.second >= bytes && e.first) return e.first;
+        if (a->base) {
+            void* p = carve(a, bytes, rc);
+            if (p) e = std::make_pair(p, bytes);
+            return p;
+        }
         if (a->sealed) {
             set_error("graph workspace slot %d would grow from %zu to %zu bytes during capture", slot, e.second,
                       bytes);
@@ -128,12 +149,49 @@ WsArena* ws_arena_create() {
     return a;
 }
 
+WsArena* ws_arena_fixed(void* base, size_t cap) {
+    WsArena* a = ws_arena_create();
+    a->base = (char*)base;
+    a->cap = cap;
+    return a;
+}
+
+void ws_arena_fixed_reset(WsArena* a, hipStream_t st) {
+    a->slots.clear();
+    a->used = 0;
+    a->st = st;
+}
+
+bool arena_fixed() { return t_arena && t_arena->base; }
+
+void* arena_upload(const void* host, size_t bytes, int* rc) {
+    *rc = OK;
+    WsArena* a = t_arena;
+    if (!a || !a->base) {
+        set_error("arena_upload outside a fixed arena");
+        *rc = E_ARG;
+        return nullptr;
+    }
+    void* p = carve(a, bytes, rc);
+    if (!p) return nullptr;
+    if (hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, a->st) != hipSuccess) {
+        set_error("table upload into the scratch failed");
+        *rc = E_LAUNCH;
+        return nullptr;
+    }
+    return p;
+}
+
 void ws_arena_seal(WsArena* a) {
     if (a) a->sealed = true;
 }
 
 void ws_arena_destroy(WsArena* a) {
     if (!a) return;
+    if (a->base) {   // fixed arenas own no memory
+        delete a;
+        return;
+    }
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(a->dev);

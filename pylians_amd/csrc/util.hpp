@@ -28,6 +28,17 @@ void ws_arena_seal(WsArena* a);
 // frees every buffer (synchronises the device first); the graph using it must be destroyed before
 void ws_arena_destroy(WsArena* a);
 size_t ws_arena_bytes(const WsArena* a);
+// A fixed arena over device memory the caller owns (the no-allocation entry points): slots and the
+// lookup tables (twiddles, windows, mode geometry) are carved from [base, base + cap) in request
+// order, and nothing is allocated; a request that does not fit fails with E_ALLOC.  Reset before each
+// call; tables are uploaded again on every call, on stream st.
+WsArena* ws_arena_fixed(void* base, size_t cap);
+void ws_arena_fixed_reset(WsArena* a, hipStream_t st);
+// true while a fixed arena is the current one on this thread
+bool arena_fixed();
+// copies `bytes` of host data into a fresh region of the current fixed arena (hipMemcpyAsync on
+// its stream; the host data must stay alive -- the table caches keep it)
+void* arena_upload(const void* host, size_t bytes, int* rc);
 struct ArenaScope {
     explicit ArenaScope(WsArena* a);
     ~ArenaScope();

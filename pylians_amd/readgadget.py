@@ -174,3 +174,19 @@ def read_block(snapshot, block, ptype, verbose=False):
     if block == "VEL " and h.redshift != 0:
         out *= np.float32(math.sqrt(h.time))
     return out
+
+
+def snapshot_files(snapshot):
+    """The files of a snapshot, in order, with their headers: [(file name, header), ...]."""
+    files, fformat = _files(snapshot)
+    if fformat == "hdf5":
+        raise Exception("HDF5 snapshots need h5py, which is not available")
+    return [(f, _Header(f)) for f in files]
+
+
+def read_segment(fname, block, ptype):
+    """Block `block` of particle type `ptype` of the single file `fname`, as stored (no velocity or
+    unit scaling; masses from the mass table where it is set)."""
+    if block not in _BLOCK_NUM:
+        raise Exception("block not implemented in readgadget!")
+    return _read_type([fname], block, int(ptype))

@@ -91,12 +91,19 @@ def test_batch64_uneven_graph(gpu):
             continue
         checked.add(j["dims"])
         host = j["pos"].cpu().numpy()
-        o = oracle_pk_particles(host, j["BoxSize"], j["dims"], "CIC", 0)
+        o, nz = C.fp64_oracle_pipeline(host, j["dims"], j["BoxSize"], 0, MAS="CIC", workers=C.host_threads(),
+                                       with_sigma=True)
         assert np.array_equal(g.Nmodes3D, o.Nmodes3D)
         assert np.allclose(g.k3D, o.k3D, rtol=1e-12, atol=0)
         e0 = _rel(g.Pk[:, 0], o.Pk[:, 0])
-        print("\nbatch job N=%d n=%d L=%.1f: P0 max rel err %.2e" % (j["dims"], len(host), j["BoxSize"], e0.max()))
-        assert e0.max() <= TOL
+        # white (shot-noise) spectra: the window-suppressed corner bins hold a few modes, where the bar is
+        # max(1e-5, 6 sigma of the float32 representation noise of the bin)
+        from tests._gpuref import NOISE_Z, _envelope
+        allowed = np.maximum(TOL, NOISE_Z * _envelope(nz.Pk[:, 0] / np.abs(o.Pk[:, 0])))
+        print("\nbatch job N=%d n=%d L=%.1f: P0 max rel err %.2e (%d of %d bins above 1e-5, max %.1f sigma)"
+              % (j["dims"], len(host), j["BoxSize"], e0.max(), int(np.sum(e0 > TOL)), len(e0),
+                 float(np.max(np.abs(g.Pk[:, 0] - o.Pk[:, 0]) / nz.Pk[:, 0]))))
+        assert np.all(e0 <= allowed)
     assert checked == {128, 192, 256, 384}
     b.close()
 

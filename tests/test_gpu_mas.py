@@ -156,6 +156,31 @@ def test_binned_is_correctly_rounded(gpu, scheme, weighted):
     assert np.allclose(g64, o, rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_binned_float32_weights(gpu, scheme, weighted):
+    """PYL_ACC_F32W: 1-D weights and products in float32 from the float64 cell fraction, float64 cell
+    sums.  Every contribution is within a few float32 ulps, so every cell (a sum of positive terms) is
+    within 1e-6 relative of the FP64 oracle, in the plain, accumulating and overdensity modes."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(32)
+    n, dims, L = 300000, 64, 200.0
+    pos = synth.zeldovich(64, L, seed=5, rms_disp=3.0, sigma_v=100.0)[:n] if not weighted else \
+        rng.random((n, 3), dtype=np.float32) * np.float32(L)
+    W = (rng.random(len(pos), dtype=np.float32) + np.float32(0.1)) if weighted else None
+    o = C.deposit_f64(pos, dims, L, scheme, W)
+    s = _lib.SCHEMES[scheme]
+    g = _dev_deposit(pos, W, dims, L, s, _lib.BINNED | _lib.ZERO | _lib.ACC_F32W)
+    rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-30)
+    assert rel.max() <= 1e-6, rel.max()
+    # the same cells through the presorted path
+    gs = _dev_deposit(pos, W, dims, L, s, _lib.SORTED | _lib.ZERO | _lib.ACC_F32W)
+    assert np.max(np.abs(gs - o) / np.maximum(np.abs(o), 1e-30)) <= 1e-6
+    start = rng.random((dims,) * 3).astype(np.float32)
+    gacc = _dev_deposit(pos, W, dims, L, s, _lib.BINNED | _lib.ACC_F32W, start=start)
+    assert np.max(np.abs(gacc - (start + o)) / (start + o)) <= 1e-6
+
+
 def test_binned_overflow_path_clustered(gpu):
     """All particles inside a few cells: buckets overflow and the exact-count path takes over."""
     from pylians_amd import _lib
