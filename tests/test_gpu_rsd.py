@@ -71,21 +71,31 @@ def test_pos_redshift_space_dropin(gpu, cols, axis):
 
 
 def test_field_div_sub_axpy_match_numpy(gpu):
-    """The overdensity and combination steps of the snapshot drivers round like numpy float32."""
+    """The field steps pyl_field_div_sub / pyl_field_axpy (the overdensity step of the snapshot drivers
+    and the Omega-weighted sum) round like numpy float32 in-place operations."""
     import torch
-    from pylians_amd import Pk_snapshot as PS
+    from pylians_amd import _lib
+
+    def div_sub(t, d, s):
+        _lib.check(_lib.lib().pyl_field_div_sub(_lib.ptr(t), int(t.numel()), float(d), float(s), _lib.F32,
+                                                _lib.stream()))
+
+    def axpy(y, x, a):
+        _lib.check(_lib.lib().pyl_field_axpy(_lib.ptr(y), _lib.ptr(x), int(y.numel()), float(a), _lib.F32,
+                                             _lib.stream()))
+
     rng = np.random.default_rng(3)
     a = (rng.random(100003) * 37.0).astype(np.float32)
     b = rng.standard_normal(100003).astype(np.float32)
     mean, om = 8.0 / 3.0 + 1e-3, 0.2718281828
     t = torch.from_numpy(a).cuda()
-    PS._div_sub(t, mean, 1.0)
+    div_sub(t, mean, 1.0)
     want = a.copy()
     want /= np.float32(mean)
     want -= np.float32(1.0)
     assert np.array_equal(t.cpu().numpy(), want)
     y = torch.from_numpy(want).cuda()
-    PS._axpy(y, torch.from_numpy(b).cuda(), om)
+    axpy(y, torch.from_numpy(b).cuda(), om)
     want2 = want + np.float32(om) * b
     assert np.array_equal(y.cpu().numpy(), want2)
 
