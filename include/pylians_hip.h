@@ -114,6 +114,18 @@ int pyl_pk_layout(int dims, int* kmax_par, int* kmax_per, int* kmax, int64_t* ou
 int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags, double* out,
            void* deltak_out, void* stream);
 
+/* No counterpart in the reference.  pyl_pk for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same arguments
+ * and results, but every scratch buffer and lookup table of the call lives in a fixed per-GPU array
+ * of PYL_NOALLOC_SCRATCH_MB MiB inside the library, so the call allocates nothing.  dims must be at
+ * most PYL_NOALLOC_MAX_DIMS (PYL_E_UNSUPPORTED otherwise).  Every call on a GPU shares that GPU's
+ * array and leaves its contents there: calls that overlap on one GPU (other host threads, or an
+ * earlier call still running on another stream) race; host threads are serialised per device. */
+#define PYL_NOALLOC_MAX_DIMS 128
+#define PYL_NOALLOC_SCRATCH_MB 64
+int pyl_pk_noalloc(const void* delta, int dims, double BoxSize, int axis, int mas_p, int prec, int flags, double* out,
+                   void* deltak_out, void* stream);
+
 /* Fused end-to-end path of Pk_snapshot.Pk_comp (Pk_snapshot.py:80-91): deposit with `scheme`,
  * delta = rho/rhobar - 1 with rhobar = sum(W)/N^3 (N_p/N^3 without weights), r2c, window
  * deconvolution with the same scheme, binning.  The density grid is not returned.  The mode check

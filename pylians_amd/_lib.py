@@ -38,6 +38,7 @@ SIGNATURES = {
     "pyl_fft_c2r": [_vp, _vp, _i, _i, _i, _vp],
     "pyl_pk_layout": [_i, _vp, _vp, _vp, _vp],
     "pyl_pk": [_vp, _i, _d, _i, _i, _i, _i, _vp, _vp, _vp],
+    "pyl_pk_noalloc": [_vp, _i, _d, _i, _i, _i, _i, _vp, _vp, _vp],
     "pyl_pk_particles": [_vp, _vp, _i64, _i, _d, _i, _i, _i, _i, _vp, _vp],
     "pyl_pk_particles_rsd": [_vp, _vp, _vp, _i64, _i, _d, _f, _f, _i, _i, _i, _i, _i, _vp, _vp],
     "pyl_pos_redshift_space": [_vp, _vp, _i64, _i, _f, _f, _f, _i, _vp],
