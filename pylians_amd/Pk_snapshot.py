@@ -30,6 +30,8 @@ from .Pk_library import XPk
 from .pipeline import PkParticles
 from .redshift_space_library import pos_redshift_space
 
+__all__ = ["Pk_comp", "Pk_Gadget", "name_dict", "rho_crit"]
+
 rho_crit = 2.77536627e11   # h^2 Msun/Mpc^3 (units_library.py:10)
 
 # file-name tags of particle types and pairs: output format of Pk_snapshot.py:20-25

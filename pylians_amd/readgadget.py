@@ -10,6 +10,8 @@ import os
 
 import numpy as np
 
+__all__ = ["header", "read_field", "read_block"]
+
 _HEADER_BYTES = 256
 _BLOCK_NUM = {"POS ": 2, "VEL ": 3, "ID  ": 4, "MASS": 5}   # readsnap.py order of the format-1 blocks
 

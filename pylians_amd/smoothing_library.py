@@ -8,6 +8,9 @@ from . import _lib
 _FILTERS = {"Top-Hat": 0, "Gaussian": 1}
 
 
+__all__ = ["FT_filter", "field_smoothing"]
+
+
 def FT_filter(BoxSize, R, dims, Filter, threads=1):
     """Fourier transform (complex64, (dims, dims, dims//2+1)) of the top-hat or Gaussian filter of
     radius R on the grid, normalised to unit sum (smoothing_library.pyx:20-86)."""

@@ -176,3 +176,35 @@ def test_xpk_oracle_consistency():
     assert np.allclose(x.XPk[:, :, 2], 2 * y.XPk[:, :, 0], rtol=1e-12)  # (2a, b)
     with pytest.raises(SystemExit):
         P.XPk([a, np.zeros((8, 8, 8))], L, MAS=["CIC", "CIC"])
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_pk1d_area_factor_white_noise(axis):
+    """Pk1D is the integral of P(k) over the disk |k_perp|^2 <= kN^2 - k_par^2 (Pk_library.pyx:387-392):
+    P1D(k_par) = int d^2k_perp / (2 pi)^2 P(k).  For white noise of variance s^2 per cell, P(k) is flat,
+    P = s^2 (L/N)^3 (and so is the measured P0), and E[P1D(k_par)] = P pi (kN^2 - k_par^2) / (2 pi)^2.
+    The mean over seeded fields matches this closed form within its sampling error; a planted area
+    factor of 2 pi kmaxper^2 / (2 pi)^2 (or pi kmaxper / (2 pi)^2) misses it."""
+    N, L, s2, R = 32, 250.0, 1.0, 24
+    rng = np.random.default_rng(77 + axis)
+    acc1, acc0, n1 = 0.0, 0.0, None
+    for _ in range(R):
+        d = rng.standard_normal((N, N, N)) * np.sqrt(s2)
+        o = P.Pk(d, L, axis=axis, MAS="None")
+        acc1 = acc1 + o.Pk1D
+        acc0 = acc0 + o.Pk[:, 0]
+        n1 = o.Nmodes1D
+        kF, kN = 2 * np.pi / L, (N // 2) * 2 * np.pi / L
+        k1 = o.k1D
+    Pflat = s2 * (L / N) ** 3
+    mean0 = acc0 / R
+    assert abs(np.mean(mean0) / Pflat - 1.0) < 0.01
+    want = Pflat * np.pi * (kN ** 2 - k1 ** 2) / (2 * np.pi) ** 2
+    mean1 = acc1 / R
+    # per bin: the mean of R * n1 exponential-like |delta_k|^2 draws, relative sd 1/sqrt(R n1)
+    ok = want > 0
+    z = (mean1[ok] - want[ok]) / (want[ok] / np.sqrt(R * n1[ok]))
+    assert np.all(np.abs(z) < 5.0), np.abs(z).max()
+    assert np.all(mean1[~ok] == 0.0)
+    wrong = Pflat * 2 * np.pi * (kN ** 2 - k1 ** 2) / (2 * np.pi) ** 2
+    assert np.max(np.abs(mean1[ok] - wrong[ok]) / (want[ok] / np.sqrt(R * n1[ok]))) > 50
