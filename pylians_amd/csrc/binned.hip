@@ -20,18 +20,17 @@
 
 #include <algorithm>
 
+#include "binned.hpp"
 #include "deposit.hpp"
 #include "profile.hpp"
 #include "util.hpp"
 
 namespace pyl {
 
-constexpr int TILE = 16;
 constexpr int BT = 512;    // k_bin: threads per workgroup
 constexpr int KP = 3;      // k_bin: particles per thread per step
 constexpr int HT = 2048;   // k_bin: LDS hash slots (> BT * KP, so an insertion always finds a slot)
 constexpr int TT = 512;    // k_tile: threads per workgroup
-constexpr int XCDS = 8;    // XCDs of gfx950; workgroup i runs on XCD i % XCDS
 static_assert(HT % BT == 0, "hash slots per thread");
 static_assert(HT > BT * KP, "every key of a step must find a hash slot");
 
@@ -140,21 +139,6 @@ __device__ __forceinline__ int wrapN(double b, int N) {
     return (unsigned)r < (unsigned)N ? r : 0;   // non-finite input: stay inside the grid
 }
 
-struct BinArgs {
-    const float* pos;
-    const float* W;
-    long long n;
-    int N, nt;           // grid edge, tiles per edge
-    double inv;          // N / BoxSize
-    unsigned* fill;      // [ntiles * 8] records per (tile, class)
-    float* bucket;       // [ntiles * CT * R]
-    long long CT;        // record slots per tile
-    int cofs[8], cap[8]; // first slot and capacity of each class segment
-    unsigned* ov_count;  // [1]
-    long long* ov_idx;   // [n]
-    unsigned* ov_key;    // [n]
-    Rsd rsd;             // redshift-space shift of the positions (rsd.vel == nullptr: none)
-};
 
 template <int S>
 __device__ __forceinline__ int halo_class(int bx, int by, int bz) {
@@ -162,7 +146,7 @@ __device__ __forceinline__ int halo_class(int bx, int by, int bz) {
 }
 
 template <int S, bool HASW>
-__global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
+__device__ __forceinline__ void bin_body(const BinArgs& a, Blk B) {
     constexpr int R = HASW ? 4 : 3;
     constexpr long long PS = (long long)BT * KP;   // particles per step
     __shared__ int hkey[HT];
@@ -201,10 +185,10 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
     };
     // each XCD (workgroups go round-robin over the XCDS of the chip) takes one contiguous range of
     // steps, so that records of one segment written at about the same time share an L2
-    const long long GX = gridDim.x / XCDS, per = (nstep + XCDS - 1) / XCDS;
-    const long long s_beg = (long long)(blockIdx.x % XCDS) * per;
+    const long long GX = B.gx / XCDS, per = (nstep + XCDS - 1) / XCDS;
+    const long long s_beg = (long long)(B.x % XCDS) * per;
     const long long s_end = min(nstep, s_beg + per);
-    long long step = s_beg + blockIdx.x / XCDS;
+    long long step = s_beg + B.x / XCDS;
     if (step < s_end) load(step);
     for (; step < s_end; step += GX) {
         float x[KP], y[KP], z[KP], w[KP];
@@ -277,17 +261,32 @@ __global__ __launch_bounds__(BT) void k_bin(BinArgs a) {
     }
 }
 
+template <int S, bool HASW>
+__global__ __launch_bounds__(BT) void k_bin(BinArgs a) { bin_body<S, HASW>(a, own_blk()); }
+template <int S, bool HASW>
+__global__ __launch_bounds__(BT) void k_bin_b(const BinArgs* __restrict__ tab,
+                                                 const unsigned* __restrict__ start, int nj) {
+    PYL_BATCH_BLK(start, nj);
+    bin_body<S, HASW>(tab[bj_], blk_);
+}
+
 // overflow grouping: count per tile, exclusive scan, scatter (records copied from the inputs)
-__global__ __launch_bounds__(256) void k_ov_count(const unsigned* __restrict__ ov_count,
-                                                   const unsigned* __restrict__ ov_key, unsigned* __restrict__ ocnt) {
-    const long long n = *ov_count;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        atomicAdd(&ocnt[ov_key[i] >> 3], 1u);
+
+__device__ __forceinline__ void ov_count_body(const OvArgs& a, Blk B) {
+    const long long n = *a.ov_count;
+    for (long long i = (long long)B.x * blockDim.x + threadIdx.x; i < n; i += (long long)B.gx * blockDim.x)
+        atomicAdd(&a.ocnt[a.ov_key[i] >> 3], 1u);
+}
+__global__ __launch_bounds__(256) void k_ov_count(OvArgs a) { ov_count_body(a, own_blk()); }
+// batched: job blockIdx.y
+__global__ __launch_bounds__(256) void k_ov_count_b(const OvArgs* __restrict__ tab) {
+    const OvArgs& a = tab[blockIdx.y];
+    ov_count_body(a, own_blk());
 }
 
 // (nothing overflowed: every group is empty and k_tile never reads an offset, so skip the scan)
-__global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ ov_count, const unsigned* __restrict__ cnt,
-                                               unsigned* __restrict__ off, int n) {
+__device__ __forceinline__ void scan_body(const unsigned* __restrict__ ov_count, const unsigned* __restrict__ cnt,
+                                          unsigned* __restrict__ off, int n) {
     __shared__ unsigned part[1024];
     if (*ov_count == 0u) return;
     const int tid = threadIdx.x;
@@ -310,30 +309,37 @@ __global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ ov_c
     }
     if (tid == 1023) off[n] = part[1023];
 }
+__global__ __launch_bounds__(1024) void k_scan(OvArgs a) { scan_body(a.ov_count, a.ocnt, a.ooff, a.ntiles); }
+// batched: one workgroup per job
+__global__ __launch_bounds__(1024) void k_scan_b(const OvArgs* __restrict__ tab) {
+    const OvArgs& a = tab[blockIdx.x];
+    scan_body(a.ov_count, a.ocnt, a.ooff, a.ntiles);
+}
 
 template <bool HASW>
-__global__ __launch_bounds__(256) void k_ov_scatter(const unsigned* __restrict__ ov_count,
-                                                     const long long* __restrict__ ov_idx,
-                                                     const unsigned* __restrict__ ov_key,
-                                                     const unsigned* __restrict__ ooff, unsigned* __restrict__ ocur,
-                                                     const float* __restrict__ pos, const float* __restrict__ W,
-                                                     float* __restrict__ grouped, unsigned* __restrict__ gcls, Rsd rsd) {
+__device__ __forceinline__ void ov_scatter_body(const OvArgs& a, Blk B) {
     constexpr int R = HASW ? 4 : 3;
-    const long long n = *ov_count;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const unsigned key = ov_key[i], t = key >> 3;
-        const long long p = ov_idx[i];
-        const unsigned s = ooff[t] + atomicAdd(&ocur[t], 1u);
-        float* r = grouped + (long long)s * R;
-        float c[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-        if (rsd.vel) c[rsd.axis] = rsd_shift(c[rsd.axis], rsd.vel[3 * p + rsd.axis], rsd.fac, rsd.box);
+    const long long n = *a.ov_count;
+    for (long long i = (long long)B.x * blockDim.x + threadIdx.x; i < n; i += (long long)B.gx * blockDim.x) {
+        const unsigned key = a.ov_key[i], t = key >> 3;
+        const long long p = a.ov_idx[i];
+        const unsigned s = a.ooff[t] + atomicAdd(&a.ocur[t], 1u);
+        float* r = a.grouped + (long long)s * R;
+        float c[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
+        if (a.rsd.vel) c[a.rsd.axis] = rsd_shift(c[a.rsd.axis], a.rsd.vel[3 * p + a.rsd.axis], a.rsd.fac, a.rsd.box);
         r[0] = c[0];
         r[1] = c[1];
         r[2] = c[2];
-        if (HASW) r[3] = W[p];
-        gcls[s] = key & 7u;
+        if (HASW) r[3] = a.W[p];
+        a.gcls[s] = key & 7u;
     }
+}
+template <bool HASW>
+__global__ __launch_bounds__(256) void k_ov_scatter(OvArgs a) { ov_scatter_body<HASW>(a, own_blk()); }
+template <bool HASW>
+__global__ __launch_bounds__(256) void k_ov_scatter_b(const OvArgs* __restrict__ tab) {
+    const OvArgs& a = tab[blockIdx.y];
+    ov_scatter_body<HASW>(a, own_blk());
 }
 
 enum { OUT_ACC = 0, OUT_SET = 1, OUT_DELTA = 2 };
@@ -348,22 +354,6 @@ __device__ __forceinline__ void put(G* __restrict__ grid, long long g, double rh
         grid[g] = (G)(rho * a - 1.0);
 }
 
-struct TileArgs {
-    const float* bucket;
-    const unsigned* fill;
-    long long CT;
-    int cofs[8], cap[8];
-    const float* grouped;
-    const unsigned* gcls;
-    const unsigned* ooff;
-    const unsigned* ocnt;
-    int N, nt;
-    double inv;
-    double mult;          // weight multiplier
-    void* grid;
-    const double* dmass;  // device mass (nullable) for OUT_DELTA
-    double mass;
-};
 
 // the stencil weights of one particle that fall inside the tile with lowest cell (ox, oy, oz)
 // (CLIP = false: a particle of the tile itself whose stencil stays inside it)
@@ -438,7 +428,7 @@ __device__ __forceinline__ void accumulate32(double* acc, const float* r, int N,
 enum { ACC_F64 = 0, ACC_F32W = 1 };
 
 template <int S, typename G, bool HASW, int MODE, bool UNIT, int ACC>
-__global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
+__device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
     constexpr int R = HASW ? 4 : 3;
     constexpr int NE = 72;   // 8 neighbours (d = 0 own) x (8 class segments + 1 overflow group)
     __shared__ double acc[TILE * TILE * TILE];
@@ -449,7 +439,7 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     // contiguous runs of tiles per XCD: the lower neighbours whose halo records a tile reads were
     // mostly processed just before on the same XCD
     const int nt = a.nt, ntl = nt * nt * nt, tper = (ntl + XCDS - 1) / XCDS;
-    const int t = (int)(blockIdx.x % XCDS) * tper + (int)(blockIdx.x / XCDS);
+    const int t = (int)(B.x % XCDS) * tper + (int)(B.x / XCDS);
     if (t >= ntl) return;
     const int tx = t / (nt * nt), ty = (t / nt) % nt, tz = t % nt;
     for (int i = tid; i < TILE * TILE * TILE; i += TT) acc[i] = 0.0;
@@ -560,28 +550,25 @@ __global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
     }
 }
 
+template <int S, typename G, bool HASW, int MODE, bool UNIT, int ACC>
+__global__ __launch_bounds__(TT) void k_tile(TileArgs a) {
+    tile_body<S, G, HASW, MODE, UNIT, ACC>(a, own_blk());
+}
+template <int S, typename G, bool HASW, int MODE, bool UNIT, int ACC>
+__global__ __launch_bounds__(TT) void k_tile_b(const TileArgs* __restrict__ tab,
+                                                  const unsigned* __restrict__ start, int nj) {
+    PYL_BATCH_BLK(start, nj);
+    tile_body<S, G, HASW, MODE, UNIT, ACC>(tab[bj_], blk_);
+}
+
 // ------------------------------------------------------------------ presort for unordered inputs
 // Particles in no spatial order give k_bin one key per particle: one global atomic each and 12-byte
 // scattered records.  The presort first groups them by supertile (tps consecutive tiles, at most
 // SUPMAX groups) with an exact count, a scan and a scatter whose per-step chunks are contiguous; k_bin
 // then reads the grouped copy, where neighbouring particles share keys as in an ordered input.
-constexpr int SUPMAX = 1024;
 constexpr int PT = 512;   // presort kernels: threads per workgroup
 constexpr int PKP = 8;    // k_sup_scatter: particles per thread per step
 
-struct SupArgs {
-    const float* pos;
-    const float* W;
-    long long n;
-    int N, nt, tps, nsup;
-    double inv;
-    Rsd rsd;
-    unsigned* cnt;   // [nsup] particles per supertile
-    unsigned* off;   // [nsup] first slot of each supertile
-    unsigned* cur;   // [nsup] slots handed out so far
-    float* spos;     // [n * 3] grouped positions (redshift-space shift applied)
-    float* sw;       // [n] grouped weights (HASW)
-};
 
 __device__ __forceinline__ void sup_load(const SupArgs& a, long long p, float& x, float& y, float& z) {
     float c[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
@@ -605,11 +592,11 @@ __device__ __forceinline__ int sup_of(const SupArgs& a, float x, float y, float 
 }
 
 template <int S>
-__global__ __launch_bounds__(PT) void k_sup_count(SupArgs a) {
+__device__ __forceinline__ void sup_count_body(const SupArgs& a, Blk B) {
     __shared__ unsigned h[SUPMAX];
     for (int i = threadIdx.x; i < a.nsup; i += PT) h[i] = 0u;
     __syncthreads();
-    for (long long p = (long long)blockIdx.x * PT + threadIdx.x; p < a.n; p += (long long)gridDim.x * PT) {
+    for (long long p = (long long)B.x * PT + threadIdx.x; p < a.n; p += (long long)B.gx * PT) {
         float x, y, z;
         sup_load(a, p, x, y, z);
         atomicAdd(&h[sup_of<S>(a, x, y, z)], 1u);
@@ -619,9 +606,17 @@ __global__ __launch_bounds__(PT) void k_sup_count(SupArgs a) {
         if (h[i]) atomicAdd(&a.cnt[i], h[i]);
 }
 
+template <int S>
+__global__ __launch_bounds__(PT) void k_sup_count(SupArgs a) { sup_count_body<S>(a, own_blk()); }
+template <int S>
+__global__ __launch_bounds__(PT) void k_sup_count_b(const SupArgs* __restrict__ tab,
+                                                       const unsigned* __restrict__ start, int nj) {
+    PYL_BATCH_BLK(start, nj);
+    sup_count_body<S>(tab[bj_], blk_);
+}
+
 // exclusive scan of the nsup <= SUPMAX counts (one workgroup of SUPMAX threads)
-__global__ __launch_bounds__(SUPMAX) void k_sup_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ off,
-                                                     int nsup) {
+__device__ __forceinline__ void sup_scan_body(const unsigned* __restrict__ cnt, unsigned* __restrict__ off, int nsup) {
     __shared__ unsigned wsum[SUPMAX / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const unsigned c = tid < nsup ? cnt[tid] : 0u;
@@ -637,16 +632,25 @@ __global__ __launch_bounds__(SUPMAX) void k_sup_scan(const unsigned* __restrict_
     for (int j = 0; j < wv; ++j) before += wsum[j];
     if (tid < nsup) off[tid] = before + v - c;
 }
+__global__ __launch_bounds__(SUPMAX) void k_sup_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ off,
+                                                     int nsup) {
+    sup_scan_body(cnt, off, nsup);
+}
+// batched: one workgroup per job
+__global__ __launch_bounds__(SUPMAX) void k_sup_scan_b(const SupArgs* __restrict__ tab) {
+    const SupArgs& a = tab[blockIdx.x];
+    sup_scan_body(a.cnt, a.off, a.nsup);
+}
 
 template <int S, bool HASW>
-__global__ __launch_bounds__(PT) void k_sup_scatter(SupArgs a) {
+__device__ __forceinline__ void sup_scatter_body(const SupArgs& a, Blk B) {
     constexpr long long PS = (long long)PT * PKP;
     __shared__ unsigned lc[SUPMAX], lb[SUPMAX];
     const int tid = threadIdx.x;
     for (int i = tid; i < a.nsup; i += PT) lc[i] = 0u;
     __syncthreads();
     const long long nstep = (a.n + PS - 1) / PS;
-    for (long long step = blockIdx.x; step < nstep; step += gridDim.x) {
+    for (long long step = B.x; step < nstep; step += B.gx) {
         float x[PKP], y[PKP], z[PKP], w[PKP];
         int sp[PKP];
         unsigned rank[PKP];
@@ -686,39 +690,68 @@ __global__ __launch_bounds__(PT) void k_sup_scatter(SupArgs a) {
     }
 }
 
+template <int S, bool HASW>
+__global__ __launch_bounds__(PT) void k_sup_scatter(SupArgs a) { sup_scatter_body<S, HASW>(a, own_blk()); }
+template <int S, bool HASW>
+__global__ __launch_bounds__(PT) void k_sup_scatter_b(const SupArgs* __restrict__ tab,
+                                                         const unsigned* __restrict__ start, int nj) {
+    PYL_BATCH_BLK(start, nj);
+    sup_scatter_body<S, HASW>(tab[bj_], blk_);
+}
+
 // ------------------------------------------------------------------ host side
 bool binned_supported(int axes, int dims) { return axes == 3 && dims % TILE == 0 && dims >= 2 * TILE; }
 
-template <int S, typename G, bool HASW, int MODE>
-static int run_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
-                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd, int acc) {
-    constexpr int R = HASW ? 4 : 3;
-    int rc = OK;
-    const int nt = dims / TILE;
-    const long long ntiles = (long long)nt * nt * nt;
-    // class segment capacities: twice the expected count of a uniform distribution plus slack
-    const double avg = (double)n / (double)ntiles;
+// Bucket layout of one deposit: class segment capacities of twice the expected count of a uniform
+// distribution plus slack, and the bytes of the counters, buckets and overflow area.
+struct Layout {
+    int nt, cap[8], cofs[8];
+    long long ntiles, CT;
+    size_t ncnt;   // counters: fill (8 ntiles) | ocnt | ocur | ooff (ntiles + 1) | ov_count
+    size_t cnt_bytes, bucket_bytes, ov_bytes;
+};
+
+static int layout_of(int64_t n, int dims, int S, bool hasw, Layout* L) {
+    const int R = hasw ? 4 : 3;
+    L->nt = dims / TILE;
+    L->ntiles = (long long)L->nt * L->nt * L->nt;
+    const double avg = (double)n / (double)L->ntiles;
     const double ph = (double)(S - 1) / TILE, p0 = 1.0 - ph;
-    int cap[8], cofs[8];
-    long long CT = 0;
+    L->CT = 0;
     for (int c = 0; c < 8; ++c) {
         const double f = ((c & 1) ? ph : p0) * ((c & 2) ? ph : p0) * ((c & 4) ? ph : p0);
         const long long v = f > 0.0 ? ((long long)(2.0 * avg * f) + 32 + 15) / 16 * 16 : 0;
         PYL_CHECK(v < (1LL << 30), E_UNSUPPORTED, "binned deposit: too many particles per tile");
-        cap[c] = (int)v;
-        cofs[c] = (int)CT;
-        CT += v;
+        L->cap[c] = (int)v;
+        L->cofs[c] = (int)L->CT;
+        L->CT += v;
     }
-    // counters: fill (8 * ntiles) | ocnt | ocur | ooff (ntiles + 1) | ov_count
-    const size_t ncnt = 11 * (size_t)ntiles + 2;
-    unsigned* cnt = (unsigned*)workspace(WS_BINS, sizeof(unsigned) * ncnt + 64, &rc);
+    L->ncnt = 11 * (size_t)L->ntiles + 2;
+    L->cnt_bytes = sizeof(unsigned) * L->ncnt + 64;
+    L->bucket_bytes = sizeof(float) * (size_t)L->ntiles * L->CT * R;
+    // overflow: ov_idx (8n) | ov_key (4n) | gcls (4n) | grouped (4Rn)
+    L->ov_bytes = (size_t)n * (16 + 4 * R) + 256;
+    return OK;
+}
+
+template <int S, typename G, bool HASW, int MODE>
+static int run_binned(const float* pos, const float* W, int64_t n, int dims, double BoxSize, double mult, void* grid,
+                      const double* dmass, double mass, hipStream_t st, const Rsd& rsd, int acc) {
+    int rc = OK;
+    Layout L;
+    rc = layout_of(n, dims, S, HASW, &L);
+    if (rc) return rc;
+    const int nt = L.nt;
+    const long long ntiles = L.ntiles, CT = L.CT;
+    const int *cap = L.cap, *cofs = L.cofs;
+    const size_t ncnt = L.ncnt;
+    unsigned* cnt = (unsigned*)workspace(WS_BINS, L.cnt_bytes, &rc);
     if (rc) return rc;
     unsigned *fill = cnt, *ocnt = cnt + 8 * ntiles, *ocur = cnt + 9 * ntiles, *ooff = cnt + 10 * ntiles,
              *ov_count = cnt + 11 * ntiles + 1;
-    float* bucket = (float*)workspace(WS_AUX, sizeof(float) * (size_t)ntiles * CT * R, &rc);
+    float* bucket = (float*)workspace(WS_AUX, L.bucket_bytes, &rc);
     if (rc) return rc;
-    // overflow: ov_idx (8n) | ov_key (4n) | gcls (4n) | grouped (4Rn)
-    char* ov = (char*)workspace(WS_AUX2, (size_t)n * (16 + 4 * R) + 256, &rc);
+    char* ov = (char*)workspace(WS_AUX2, L.ov_bytes, &rc);
     if (rc) return rc;
     long long* ov_idx = (long long*)ov;
     unsigned* ov_key = (unsigned*)(ov + 8 * (size_t)n);
@@ -741,10 +774,10 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
     }
     {
         StageTimer tm("deposit_overflow", st);
-        hipLaunchKernelGGL(k_ov_count, dim3(256), dim3(256), 0, st, ov_count, ov_key, ocnt);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, ov_count, ocnt, ooff, (int)ntiles);
-        hipLaunchKernelGGL((k_ov_scatter<HASW>), dim3(256), dim3(256), 0, st, ov_count, ov_idx, ov_key, ooff, ocur,
-                           pos, W, grouped, gcls, rsd);
+        const OvArgs oa{ov_count, ov_idx, ov_key, ocnt, ooff, ocur, pos, W, grouped, gcls, rsd, (int)ntiles};
+        hipLaunchKernelGGL(k_ov_count, dim3(256), dim3(256), 0, st, oa);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, oa);
+        hipLaunchKernelGGL((k_ov_scatter<HASW>), dim3(256), dim3(256), 0, st, oa);
         PYL_HIP(hipGetLastError());
     }
     {
@@ -861,6 +894,144 @@ int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double
     if (mode == OUT_ACC) return dispatch_s<float, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
     if (mode == OUT_SET) return dispatch_s<float, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
     return dispatch_s<float, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
+}
+
+// ------------------------------------------------------------------ batched deposit
+// Jobs of a batch (pyl_graph_create / pyl_pk_batch) run each stage of the deposit as ONE launch for
+// all jobs of a class (same scheme, weights, grid type, accumulation and presort): the arguments of
+// job j sit in device tables and its blocks are the range [start[j], start[j+1]) of the launch.
+static size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
+static void sup_geometry(int dims, int* tps, int* nsup) {
+    const int nt = dims / TILE;
+    const long long ntiles = (long long)nt * nt * nt;
+    long long t = nt * std::max(1LL, (ntiles + 128LL * nt - 1) / (128LL * nt));
+    while ((ntiles + t - 1) / t > SUPMAX) t += nt;
+    *tps = (int)t;
+    *nsup = (int)((ntiles + t - 1) / t);
+}
+
+size_t binned_job_bytes(int64_t n, int dims, int scheme, bool hasw, bool sorted) {
+    Layout L;
+    if (!binned_supported(3, dims) || layout_of(n, dims, scheme, hasw, &L) != OK) return 0;
+    size_t b = al256(L.cnt_bytes) + al256(L.bucket_bytes) + al256(L.ov_bytes);
+    if (sorted) b += al256(sizeof(unsigned) * 3 * SUPMAX) + al256(sizeof(float) * (size_t)n * (hasw ? 4 : 3) + 64);
+    return b;
+}
+
+int binned_job_plan(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, void* grid,
+                    const double* dmass, double mass, bool sorted, char* mem, BinnedJob* j) {
+    PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs 3-D grids with dims %% %d == 0", TILE);
+    PYL_CHECK(n > 0 && n < (1LL << 31), E_UNSUPPORTED, "binned deposit: 1 to 2^31 - 1 particles per job");
+    const bool hasw = W != nullptr;
+    const int R = hasw ? 4 : 3;
+    Layout L;
+    int rc = layout_of(n, dims, scheme, hasw, &L);
+    if (rc) return rc;
+    char* p = mem;
+    unsigned* cnt = (unsigned*)p;
+    p += al256(L.cnt_bytes);
+    float* bucket = (float*)p;
+    p += al256(L.bucket_bytes);
+    char* ov = p;
+    p += al256(L.ov_bytes);
+    const long long ntiles = L.ntiles;
+    unsigned *fill = cnt, *ocnt = cnt + 8 * ntiles, *ocur = cnt + 9 * ntiles, *ooff = cnt + 10 * ntiles,
+             *ov_count = cnt + 11 * ntiles + 1;
+    long long* ov_idx = (long long*)ov;
+    unsigned* ov_key = (unsigned*)(ov + 8 * (size_t)n);
+    unsigned* gcls = (unsigned*)(ov + 12 * (size_t)n);
+    float* grouped = (float*)(ov + 16 * (size_t)n + 64);
+    const double inv = dims / BoxSize;
+    const float* bpos = pos;
+    const float* bW = W;
+    j->sup = SupArgs{};
+    j->sup_blocks = j->sup_sblocks = 0;
+    j->scnt = nullptr;
+    j->scnt_bytes = 0;
+    if (sorted) {
+        unsigned* sc = (unsigned*)p;
+        p += al256(sizeof(unsigned) * 3 * SUPMAX);
+        float* sp = (float*)p;
+        int tps = 0, nsup = 0;
+        sup_geometry(dims, &tps, &nsup);
+        j->sup = SupArgs{pos, W, (long long)n, dims, L.nt, tps, nsup, inv, Rsd(), sc, sc + SUPMAX, sc + 2 * SUPMAX,
+                         sp, hasw ? sp + 3 * n : nullptr};
+        j->sup_blocks = (unsigned)std::min<long long>(2048, (n + PT - 1) / PT);
+        j->sup_sblocks = (unsigned)std::min<long long>(2048, (n + (long long)PT * PKP - 1) / ((long long)PT * PKP));
+        j->scnt = sc;
+        j->scnt_bytes = sizeof(unsigned) * 3 * SUPMAX;
+        bpos = sp;
+        bW = j->sup.sw;
+    }
+    (void)R;
+    j->bin = BinArgs{bpos, bW, (long long)n, dims, L.nt, inv, fill, bucket, L.CT, {}, {}, ov_count, ov_idx, ov_key,
+                     Rsd()};
+    j->tile = TileArgs{bucket, fill, L.CT, {}, {}, grouped, gcls, ooff, ocnt, dims, L.nt, inv, 1.0, grid, dmass, mass};
+    for (int c = 0; c < 8; ++c) {
+        j->bin.cofs[c] = j->tile.cofs[c] = L.cofs[c];
+        j->bin.cap[c] = j->tile.cap[c] = L.cap[c];
+    }
+    long long bb = (n + (long long)BT * KP - 1) / ((long long)BT * KP);
+    if (bb > 2048) bb = 2048;
+    j->bin_blocks = (unsigned)((bb + XCDS - 1) / XCDS * XCDS);
+    j->ov = OvArgs{ov_count, ov_idx, ov_key, ocnt, ooff, ocur, bpos, bW, grouped, gcls, Rsd(), (int)ntiles};
+    j->tile_blocks = (unsigned)((ntiles + XCDS - 1) / XCDS * XCDS);
+    j->cnt = cnt;
+    j->cnt_bytes = sizeof(unsigned) * L.ncnt;
+    return OK;
+}
+
+template <int S, bool HASW, typename G, int ACC>
+static int batch_launch_s(bool sorted, const BinnedTables& t, hipStream_t st) {
+    const unsigned nj = (unsigned)t.nj;
+    if (sorted) {
+        StageTimer tm("deposit_presort", st);
+        hipLaunchKernelGGL((k_sup_count_b<S>), dim3(t.sup_total), dim3(PT), 0, st, t.sup, t.sup_start, t.nj);
+        hipLaunchKernelGGL(k_sup_scan_b, dim3(nj), dim3(SUPMAX), 0, st, t.sup);
+        hipLaunchKernelGGL((k_sup_scatter_b<S, HASW>), dim3(t.ssc_total), dim3(PT), 0, st, t.sup, t.ssc_start, t.nj);
+        PYL_HIP(hipGetLastError());
+    }
+    {
+        StageTimer tm("deposit_bin", st);
+        hipLaunchKernelGGL((k_bin_b<S, HASW>), dim3(t.bin_total), dim3(BT), 0, st, t.bin, t.bin_start, t.nj);
+        PYL_HIP(hipGetLastError());
+    }
+    {
+        StageTimer tm("deposit_overflow", st);
+        hipLaunchKernelGGL(k_ov_count_b, dim3(256, nj), dim3(256), 0, st, t.ov);
+        hipLaunchKernelGGL(k_scan_b, dim3(nj), dim3(1024), 0, st, t.ov);
+        hipLaunchKernelGGL((k_ov_scatter_b<HASW>), dim3(256, nj), dim3(256), 0, st, t.ov);
+        PYL_HIP(hipGetLastError());
+    }
+    {
+        StageTimer tm("deposit_tiles", st);
+        hipLaunchKernelGGL((k_tile_b<S, G, HASW, OUT_DELTA, !HASW, ACC>), dim3(t.tile_total), dim3(TT), 0, st, t.tile,
+                           t.tile_start, t.nj);
+        PYL_HIP(hipGetLastError());
+    }
+    return OK;
+}
+
+template <int S>
+static int batch_launch_w(bool hasw, bool f64, int acc, bool sorted, const BinnedTables& t, hipStream_t st) {
+    if (f64) return hasw ? batch_launch_s<S, true, double, ACC_F64>(sorted, t, st)
+                         : batch_launch_s<S, false, double, ACC_F64>(sorted, t, st);
+    if (acc == ACC_F32W) return hasw ? batch_launch_s<S, true, float, ACC_F32W>(sorted, t, st)
+                                     : batch_launch_s<S, false, float, ACC_F32W>(sorted, t, st);
+    return hasw ? batch_launch_s<S, true, float, ACC_F64>(sorted, t, st)
+                : batch_launch_s<S, false, float, ACC_F64>(sorted, t, st);
+}
+
+int binned_batch_launch(int scheme, bool hasw, bool f64, int acc, bool sorted, const BinnedTables& t, hipStream_t st) {
+    if (t.nj <= 0) return OK;
+    switch (scheme) {
+        case 1: return batch_launch_w<1>(hasw, f64, acc, sorted, t, st);
+        case 2: return batch_launch_w<2>(hasw, f64, acc, sorted, t, st);
+        case 3: return batch_launch_w<3>(hasw, f64, acc, sorted, t, st);
+        case 4: return batch_launch_w<4>(hasw, f64, acc, sorted, t, st);
+        default: set_error("unknown scheme %d", scheme); return E_ARG;
+    }
 }
 
 }  // namespace pyl

@@ -236,7 +236,6 @@ extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int
 // supertile once (all overwrite the grid, so the trial runs are harmless), keeps the fastest and
 // caches the choice.  pyl_tune_export / pyl_tune_import move the cache as text ("dims scheme prec
 // hasw nlog choice" lines; choice 0 = binned, 1 = atomic, 2 = presorted binned).
-enum DepositKind { DK_BINNED = 0, DK_ATOMIC = 1, DK_SORTED = 2 };
 namespace {
 struct TuneKey {
     int dev, dims, scheme, prec, hasw, nlog;
@@ -356,8 +355,23 @@ extern "C" int pyl_tune_import(const char* text) {
     return OK;
 }
 
-static int pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
-                        int prec, int flags, double* out, hipStream_t st, const Rsd& rsd) {
+namespace pyl {
+int deposit_kind(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
+                 int flags, void* grid, const double* dmass, hipStream_t st, int* kind, const Rsd& rsd) {
+    if (flags & DEP_ATOMIC) {
+        *kind = DK_ATOMIC;
+        return OK;
+    }
+    if (flags & (DEP_BINNED | DEP_SORTED)) {
+        PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
+        *kind = (flags & DEP_SORTED) ? DK_SORTED : DK_BINNED;
+        return OK;
+    }
+    return tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, kind, rsd, flags);
+}
+
+int pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
+                 int prec, int flags, double* out, hipStream_t st, const Rsd& rsd) {
     int rc = check_fft(dims, prec);
     if (rc) return rc;
     PYL_CHECK(axis >= 0 && axis <= 2, E_ARG, "axis must be 0, 1 or 2");
@@ -377,15 +391,8 @@ static int pk_particles(const float* pos, const float* W, int64_t n, int dims, d
     // binned path: the deposit writes delta = rho/rhobar - 1 itself (float64 before one rounding);
     // atomic path: rho, normalised inside pass Z
     int kind;
-    if (flags & DEP_ATOMIC) {
-        kind = DK_ATOMIC;
-    } else if (flags & (DEP_BINNED | DEP_SORTED)) {
-        PYL_CHECK(binned_supported(3, dims), E_UNSUPPORTED, "binned deposit needs dims %% 16 == 0");
-        kind = (flags & DEP_SORTED) ? DK_SORTED : DK_BINNED;
-    } else {
-        rc = tune_choice(pos, W, n, dims, BoxSize, scheme, f64, grid, dmass, st, &kind, rsd, flags);
-        if (rc) return rc;
-    }
+    rc = deposit_kind(pos, W, n, dims, BoxSize, scheme, f64, flags, grid, dmass, st, &kind, rsd);
+    if (rc) return rc;
     rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, kind, grid, dmass, st, rsd, flags);
     if (rc) return rc;
     const int pkflags = flags & (PYL_PK_ALL | PYL_PK_ASYNC | PYL_PK_DROP_KZ0);
@@ -395,6 +402,7 @@ static int pk_particles(const float* pos, const float* W, int64_t n, int dims, d
     return pk_from_delta<float>((const float*)grid, dims, BoxSize, axis, scheme, pkflags, out, nullptr, false,
                                 dmass, (double)n, st);
 }
+}  // namespace pyl
 
 extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
                                 int axis, int prec, int flags, double* out, void* stream) {
@@ -478,91 +486,3 @@ extern "C" int pyl_interp(const float* density, int dims, double BoxSize, const 
     return interp(density, dims, BoxSize, pos, n, scheme, out, (hipStream_t)stream);
 }
 
-// ------------------------------------------------------------------ batches and graphs
-extern "C" int pyl_pk_batch(const pyl_pk_job* jobs, int njobs, void* stream) {
-    PYL_CHECK(njobs >= 0 && (njobs == 0 || jobs), E_ARG, "bad job array");
-    for (int i = 0; i < njobs; ++i) {
-        const pyl_pk_job& j = jobs[i];
-        int rc = pyl_pk_particles(j.pos, j.W, j.n, j.dims, j.BoxSize, j.scheme, j.axis, j.prec, j.flags | PYL_PK_ASYNC,
-                                  j.out, stream);
-        if (rc) return rc;
-    }
-    return OK;
-}
-
-namespace {
-// A captured batch and the workspaces it owns: the graph's kernels hold pointers into the arena,
-// so later eager calls (which grow and free the shared workspaces) cannot invalidate them.
-struct GraphHandle {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    WsArena* arena = nullptr;
-};
-
-void graph_free(GraphHandle* h) {
-    if (h->exec) (void)hipGraphExecDestroy(h->exec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
-    ws_arena_destroy(h->arena);
-    delete h;
-}
-}  // namespace
-
-extern "C" int pyl_graph_create(const pyl_pk_job* jobs, int njobs, void* stream, void** graph) {
-    PYL_CHECK(graph, E_ARG, "null graph handle");
-    hipStream_t st = (hipStream_t)stream;
-    PYL_CHECK(st != nullptr, E_ARG, "graph capture needs a non-default stream");
-    GraphHandle* h = new GraphHandle();
-    h->arena = ws_arena_create();
-    int rc = OK;
-    {
-        ArenaScope scope(h->arena);
-        // warm run: sizes every workspace of the arena, builds twiddle / window / geometry tables
-        // and settles the deposit choices
-        rc = pyl_pk_batch(jobs, njobs, stream);
-        hipError_t e = rc ? hipSuccess : hipStreamSynchronize(st);
-        if (rc == OK && e != hipSuccess) {
-            set_error("hipStreamSynchronize -> %s", hipGetErrorString(e));
-            rc = (int)e + 1000;
-        }
-        if (rc == OK) {
-            ws_arena_seal(h->arena);
-            e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
-            if (e != hipSuccess) {
-                set_error("hipStreamBeginCapture -> %s", hipGetErrorString(e));
-                rc = (int)e + 1000;
-            } else {
-                rc = pyl_pk_batch(jobs, njobs, stream);
-                e = hipStreamEndCapture(st, &h->graph);
-                if (rc == OK && e != hipSuccess) {
-                    set_error("hipStreamEndCapture -> %s", hipGetErrorString(e));
-                    rc = (int)e + 1000;
-                }
-            }
-        }
-    }
-    if (rc == OK) {
-        hipError_t e = hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            set_error("hipGraphInstantiate -> %s", hipGetErrorString(e));
-            rc = (int)e + 1000;
-        }
-    }
-    if (rc) {
-        graph_free(h);
-        return rc;
-    }
-    *graph = h;
-    return OK;
-}
-
-extern "C" int pyl_graph_launch(void* graph, void* stream) {
-    PYL_CHECK(graph, E_ARG, "null graph");
-    PYL_HIP(hipGraphLaunch(((GraphHandle*)graph)->exec, (hipStream_t)stream));
-    return OK;
-}
-
-extern "C" int pyl_graph_destroy(void* graph) {
-    if (!graph) return OK;
-    graph_free((GraphHandle*)graph);
-    return OK;
-}

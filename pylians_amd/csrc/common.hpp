@@ -72,4 +72,28 @@ __device__ __forceinline__ int kfreq(int i, int n) { return i > n / 2 ? i - n : 
 
 inline int64_t half_len(int n) { return (int64_t)(n / 2 + 1); }
 
+// ---------------------------------------------------------------- block coordinates
+// Kernel bodies take their block index and block count explicitly: a single launch passes
+// (blockIdx.x, gridDim.x); a batched launch (one launch for the same stage of many jobs) gives every
+// job a contiguous range of blocks and passes the index and count within the job.
+struct Blk {
+    unsigned x, gx;
+};
+__device__ __forceinline__ Blk own_blk() { return Blk{blockIdx.x, gridDim.x}; }
+// job of block b of a batched launch: job j owns blocks [start[j], start[j+1]); start[] ascends and
+// every start is a multiple of the XCD count, so a job's block i runs on XCD i % 8 as in a single launch
+__device__ __forceinline__ int batch_job(const unsigned* __restrict__ start, int nj, unsigned b) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// the batched counterpart of a body taking (const Args&, Blk): job tab[j], block range from start
+#define PYL_BATCH_BLK(start, nj)                                          \
+    const int bj_ = ::pyl::batch_job(start, nj, blockIdx.x);              \
+    const ::pyl::Blk blk_{blockIdx.x - start[bj_], start[bj_ + 1] - start[bj_]}
+
 }  // namespace pyl

@@ -16,11 +16,16 @@ namespace pyl {
 // M-point FFT in LDS, then X[k] = E[k] + W_N^k O[k] for k in [0, M].  Persistent blocks walk row
 // blocks; the next block's rows are loaded (coalesced along the rows) into registers while the
 // current one is transformed.
+
 template <typename R, int M, int NL, int T>
-__global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R>* __restrict__ out,
-                                                 long long nrows, long long P,
-                                                 const cplx<R>* __restrict__ twg, int affine,
-                                                 const double* __restrict__ dmass, double mass, double ncell) {
+__device__ __forceinline__ void r2c_rows_body(const R2cArgs<R>& A, Blk B) {
+    const R* __restrict__ in = A.in;
+    cplx<R>* __restrict__ out = A.out;
+    const long long nrows = A.nrows, P = A.P;
+    const cplx<R>* __restrict__ twg = A.tw;
+    const int affine = A.affine;
+    const double* __restrict__ dmass = A.dmass;
+    const double mass = A.mass, ncell = A.ncell;
     constexpr int N = 2 * M;
     constexpr int ES = NL + 1;
     constexpr int V = (NL * M + T - 1) / T;   // complex inputs per thread per row block
@@ -49,13 +54,13 @@ __global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R
             nxt[q] = v;
         }
     };
-    long long rb = blockIdx.x;
+    long long rb = B.x;
     if (rb < nrb) load(rb);
-    for (; rb < nrb; rb += gridDim.x) {
+    for (; rb < nrb; rb += B.gx) {
         cplx<R> cur[V];
 #pragma unroll
         for (int q = 0; q < V; ++q) cur[q] = nxt[q];
-        if (rb + gridDim.x < nrb) load(rb + gridDim.x);
+        if (rb + B.gx < nrb) load(rb + B.gx);
         __syncthreads();   // the previous row block's post-processing has read s
 #pragma unroll
         for (int q = 0; q < V; ++q) {
@@ -84,6 +89,16 @@ __global__ __launch_bounds__(T) void k_r2c_rows(const R* __restrict__ in, cplx<R
             out[row * P + k] = e + cmul(tw[k], o);
         }
     }
+}
+template <typename R, int M, int NL, int T>
+__global__ __launch_bounds__(T) void k_r2c_rows(R2cArgs<R> a) {
+    r2c_rows_body<R, M, NL, T>(a, own_blk());
+}
+template <typename R, int M, int NL, int T>
+__global__ __launch_bounds__(T) void k_r2c_rows_b(const R2cArgs<R>* __restrict__ tab, const unsigned* __restrict__ start,
+                                                   int nj) {
+    PYL_BATCH_BLK(start, nj);
+    r2c_rows_body<R, M, NL, T>(tab[bj_], blk_);
 }
 
 // Pass Z inverse: rows of M+1 complex -> N = 2M reals.
@@ -211,10 +226,15 @@ __global__ __launch_bounds__(1024) void k_c2r_partsum(const double* __restrict__
 // Persistent blocks walk column blocks of NL columns; while one block is transformed the first-stage
 // inputs of the next one are already in flight into registers, and the last stage stores straight
 // from registers.  With a single radix stage the transform never touches LDS.
+
 template <typename R, int N, int NL, int T, bool INV>
-__global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, long long ncols,
-                                             long long CW, long long CS, long long EST, long long PW, int W,
-                                             const cplx<R>* __restrict__ twg, R sc) {
+__device__ __forceinline__ void cols_body(const ColArgs<R>& A, Blk B) {
+    const cplx<R>* src = A.src;
+    cplx<R>* dst = A.dst;
+    const long long ncols = A.ncols, CW = A.CW, CS = A.CS, EST = A.EST, PW = A.PW;
+    const int W = A.W;
+    const cplx<R>* __restrict__ twg = A.tw;
+    const R sc = A.sc;
     static_assert(T % NL == 0, "T must be a multiple of NL");
     using F = First<N, NL, T>;
     // prefetching doubles the first stage's registers; beyond 512 points it would spill
@@ -231,7 +251,7 @@ __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, lo
         return valid ? (c / CW) * CS + (c % CW) : 0;
     };
     cplx<R> nxt[F::B][F::RAD];
-    long long cb = blockIdx.x;
+    long long cb = B.x;
     if (PREFETCH && cb < nblk) {
         bool v;
         const long long b = col_base(cb, v);
@@ -240,7 +260,7 @@ __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, lo
             return INV ? conjg(x) : x;
         }, tid);
     }
-    for (; cb < nblk; cb += gridDim.x) {
+    for (; cb < nblk; cb += B.gx) {
         cplx<R> cur[F::B][F::RAD];
         if (PREFETCH) {
 #pragma unroll
@@ -255,7 +275,7 @@ __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, lo
                 return INV ? conjg(x) : x;
             }, tid);
         }
-        const long long cbn = cb + gridDim.x;
+        const long long cbn = cb + B.gx;
         if (PREFETCH && cbn < nblk) {
             bool v;
             const long long b = col_base(cbn, v);
@@ -287,6 +307,16 @@ __global__ __launch_bounds__(T) void k_cols(const cplx<R>* src, cplx<R>* dst, lo
             last_stage_store<R, N, NL, NL, T, 1>(s, tw, store, tid);
         }
     }
+}
+template <typename R, int N, int NL, int T, bool INV>
+__global__ __launch_bounds__(T) void k_cols(ColArgs<R> a) {
+    cols_body<R, N, NL, T, INV>(a, own_blk());
+}
+template <typename R, int N, int NL, int T, bool INV>
+__global__ __launch_bounds__(T) void k_cols_b(const ColArgs<R>* __restrict__ tab, const unsigned* __restrict__ start,
+                                               int nj) {
+    PYL_BATCH_BLK(start, nj);
+    cols_body<R, N, NL, T, INV>(tab[bj_], blk_);
 }
 
 // ----------------------------------------------------------------------- dispatch
@@ -359,25 +389,49 @@ template <typename R> const cplx<R>* twiddles(int N) {
 template const cplx<float>* twiddles<float>(int);
 template const cplx<double>* twiddles<double>(int);
 
+// pass Z launch shape of one job: M = N/2-point complex rows, NL rows per block; persistent grid of
+// about as many blocks as fit on the chip at once (LDS bound)
+template <typename R, int N> struct R2cShape {
+    static constexpr int M = N / 2;
+    static constexpr int NL = Geo<R>::NL;
+    // rows with a radix-3 factor: 8 instead of 16 complex values per thread measured faster
+    static constexpr int T = threads_for(M, NL) * ((M % 3 == 0 && M >= 96) ? 2 : 1);
+    static constexpr int LDS = (M * (NL + 1) + N) * (int)sizeof(cplx<R>);
+};
+
+template <typename R, int N>
+static int r2c_job_n(const R* in, cplx<R>* out, long long nrows, int64_t P, bool affine, const double* dmass,
+                     double mass, R2cArgs<R>* a, unsigned* blocks) {
+    using SH = R2cShape<R, N>;
+    const cplx<R>* tw = twiddles<R>(N);
+    PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
+    int per_cu = (160 * 1024) / (SH::LDS + 1024);
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    long long nb = (long long)cu_count() * per_cu;
+    const long long nrb = (nrows + SH::NL - 1) / SH::NL;
+    if (nb > nrb) nb = nrb;
+    *a = R2cArgs<R>{in, out, nrows, (long long)P, tw, affine ? 1 : 0, dmass, mass, (double)N * N * N};
+    *blocks = (unsigned)nb;
+    return OK;
+}
+
 template <typename R, int N>
 static int launch_r2c_rows(const R* in, cplx<R>* out, long long nrows, int64_t P, bool affine,
                            const double* dmass, double mass, hipStream_t st) {
-    constexpr int M = N / 2;
-    constexpr int NL = Geo<R>::NL;
-    // rows with a radix-3 factor: 8 instead of 16 complex values per thread measured faster
-    constexpr int T = threads_for(M, NL) * ((M % 3 == 0 && M >= 96) ? 2 : 1);
-    const cplx<R>* tw = twiddles<R>(N);
-    PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
-    // persistent grid: about as many blocks as fit on the chip at once (LDS bound)
-    constexpr int lds = (M * (NL + 1) + N) * (int)sizeof(cplx<R>);
-    int per_cu = (160 * 1024) / (lds + 1024);
-    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
-    long long nb = (long long)cu_count() * per_cu;
-    const long long nrb = (nrows + NL - 1) / NL;
-    if (nb > nrb) nb = nrb;
-    dim3 grid((unsigned)nb);
-    hipLaunchKernelGGL((k_r2c_rows<R, M, NL, T>), grid, dim3(T), 0, st, in, out, nrows, (long long)P, tw,
-                       affine ? 1 : 0, dmass, mass, (double)N * N * N);
+    using SH = R2cShape<R, N>;
+    R2cArgs<R> a;
+    unsigned nb = 0;
+    int rc = r2c_job_n<R, N>(in, out, nrows, P, affine, dmass, mass, &a, &nb);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_r2c_rows<R, SH::M, SH::NL, SH::T>), dim3(nb), dim3(SH::T), 0, st, a);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R, int N>
+static int launch_r2c_batch(const R2cArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st) {
+    using SH = R2cShape<R, N>;
+    hipLaunchKernelGGL((k_r2c_rows_b<R, SH::M, SH::NL, SH::T>), dim3(total), dim3(SH::T), 0, st, tab, start, nj);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -407,12 +461,17 @@ static int launch_c2r_rows(const cplx<R>* in, R* out, long long nrows, int64_t P
     return OK;
 }
 
-template <typename R, int N>
-static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st,
-                       int K) {
-    constexpr int NL = (N >= 1024 && sizeof(R) == 4) ? 8 : Geo<R>::NL;   // 1024-point lines: 8 per block
+template <typename R, int N> struct ColShape {
+    static constexpr int NL = (N >= 1024 && sizeof(R) == 4) ? 8 : Geo<R>::NL;   // 1024-point lines: 8 per block
     // 16 elements per thread (the prefetch doubles the first stage's registers)
-    constexpr int T = clamp_i(N * NL / 16, 64, 512);
+    static constexpr int T = clamp_i(N * NL / 16, 64, 512);
+    static constexpr int LDS = (N * NL + N) * (int)sizeof(cplx<R>);
+};
+
+template <typename R, int N>
+static int cols_job_n(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, R sc, int K, ColArgs<R>* a,
+                      unsigned* blocks) {
+    using SH = ColShape<R, N>;
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
     // K < W (pass along y only): just the columns kz < K, enumerated without the others
@@ -430,20 +489,37 @@ static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bo
         CS = 0;
         EST = (long long)N * P;
     }
-    const long long nblk = (ncols + NL - 1) / NL;
+    const long long nblk = (ncols + SH::NL - 1) / SH::NL;
     // persistent grid: about as many blocks as fit on the chip at once (LDS bound)
-    constexpr int lds = (N * NL + N) * (int)sizeof(cplx<R>);
-    int per_cu = (160 * 1024) / (lds + 1024);
+    int per_cu = (160 * 1024) / (SH::LDS + 1024);
     per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
     long long nb = (long long)cu_count() * per_cu;
     if (nb > nblk) nb = nblk;
-    dim3 grid((unsigned)nb);
+    *a = ColArgs<R>{src, dst, ncols, CW, CS, EST, PW, WV, tw, sc};
+    *blocks = (unsigned)nb;
+    return OK;
+}
+
+template <typename R, int N>
+static int launch_cols(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, bool inv, R sc, hipStream_t st,
+                       int K) {
+    using SH = ColShape<R, N>;
+    ColArgs<R> a;
+    unsigned nb = 0;
+    int rc = cols_job_n<R, N>(src, dst, P, axis, sc, K, &a, &nb);
+    if (rc) return rc;
     if (inv)
-        hipLaunchKernelGGL((k_cols<R, N, NL, T, true>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, PW, WV,
-                           tw, sc);
+        hipLaunchKernelGGL((k_cols<R, N, SH::NL, SH::T, true>), dim3(nb), dim3(SH::T), 0, st, a);
     else
-        hipLaunchKernelGGL((k_cols<R, N, NL, T, false>), grid, dim3(T), 0, st, src, dst, ncols, CW, CS, EST, PW, WV,
-                           tw, sc);
+        hipLaunchKernelGGL((k_cols<R, N, SH::NL, SH::T, false>), dim3(nb), dim3(SH::T), 0, st, a);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R, int N>
+static int launch_cols_batch(const ColArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st) {
+    using SH = ColShape<R, N>;
+    hipLaunchKernelGGL((k_cols_b<R, N, SH::NL, SH::T, false>), dim3(total), dim3(SH::T), 0, st, tab, start, nj);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -513,12 +589,73 @@ int fft3d_c2r(const cplx<R>* in, R* out, cplx<R>* ws, int N, int64_t P, R sc, hi
     return fft_pass_z_c2r<R>(ws, out, N, P, sc, 1, nullptr, nullptr, st);
 }
 
+template <typename R>
+int r2c_job(const R* in, cplx<R>* out, int N, int64_t P, bool affine, const double* dmass, double mass,
+            R2cArgs<R>* a, unsigned* blocks) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return r2c_job_n<R, n>(in, out, (long long)N * N, P, affine, dmass, mass, a, blocks);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int r2c_batch(int N, const R2cArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_r2c_batch<R, n>(tab, start, nj, total, st);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int cols_job(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, ColArgs<R>* a, unsigned* blocks) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return cols_job_n<R, n>(src, dst, P, axis, (R)1, 0, a, blocks);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int cols_batch(int N, const ColArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st) {
+    switch (N) {
+#define PYL_CASE(n) \
+    case n:         \
+        return launch_cols_batch<R, n>(tab, start, nj, total, st);
+        PYL_FFT_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("FFT size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
 #define PYL_INST(R)                                                                                        \
     template int fft_pass_z_r2c<R>(const R*, cplx<R>*, int, int64_t, bool, const double*, double, hipStream_t);               \
     template int fft_pass_cols<R>(const cplx<R>*, cplx<R>*, int, int64_t, int, bool, R, hipStream_t, int);    \
     template int fft_pass_z_c2r<R>(const cplx<R>*, R*, int, int64_t, R, int, const R*, double*, hipStream_t, int); \
     template int fft3d_r2c<R>(const R*, cplx<R>*, int, int64_t, hipStream_t);                                \
-    template int fft3d_c2r<R>(const cplx<R>*, R*, cplx<R>*, int, int64_t, R, hipStream_t);
+    template int fft3d_c2r<R>(const cplx<R>*, R*, cplx<R>*, int, int64_t, R, hipStream_t);                   \
+    template int r2c_job<R>(const R*, cplx<R>*, int, int64_t, bool, const double*, double, R2cArgs<R>*, unsigned*); \
+    template int r2c_batch<R>(int, const R2cArgs<R>*, const unsigned*, int, unsigned, hipStream_t);          \
+    template int cols_job<R>(const cplx<R>*, cplx<R>*, int, int64_t, int, ColArgs<R>*, unsigned*);          \
+    template int cols_batch<R>(int, const ColArgs<R>*, const unsigned*, int, unsigned, hipStream_t);
 PYL_INST(float)
 PYL_INST(double)
 #undef PYL_INST

@@ -36,6 +36,39 @@ template <typename R>
 int fft_pass_z_c2r(const cplx<R>* in, R* out, int N, int64_t P, R scale, int store, const R* aux,
                    double* acc, hipStream_t st, int K = 0);
 
+// Argument blocks of pass Z (rows r2c) and of the column pass, shared with the batch planner
+template <typename R> struct R2cArgs {
+    const R* in;
+    cplx<R>* out;
+    long long nrows, P;
+    const cplx<R>* tw;
+    int affine;
+    const double* dmass;
+    double mass, ncell;
+};
+
+template <typename R> struct ColArgs {
+    const cplx<R>* src;
+    cplx<R>* dst;
+    long long ncols, CW, CS, EST, PW;
+    int W;
+    const cplx<R>* tw;
+    R sc;
+};
+
+// Batched launches over jobs of one grid size N (one launch for all of them): *_job builds a job's
+// arguments and block count (no launch), *_batch launches the device table tab[0..nj) whose
+// block ranges are start[0..nj] (start[nj] = total).
+template <typename R>
+int r2c_job(const R* in, cplx<R>* out, int N, int64_t P, bool affine, const double* dmass, double mass,
+            R2cArgs<R>* a, unsigned* blocks);
+template <typename R>
+int r2c_batch(int N, const R2cArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st);
+template <typename R>
+int cols_job(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, ColArgs<R>* a, unsigned* blocks);
+template <typename R>
+int cols_batch(int N, const ColArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st);
+
 // full transforms
 template <typename R>
 int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st);

@@ -355,12 +355,18 @@ __device__ __forceinline__ void last_stage_mapped(cplx<R>* s, const cplx<R>* tw,
 // Every histogram entry also counts the modes it received (Nmodes, Pk_library.pyx:376-381): a
 // block's row of `part` is [3-D sums (4 n3) | 1-D sums (n1) | 3-D counts (n3) | 1-D counts (n1) |
 // 2-D sums (n2w) | 2-D counts (n2w)], counts held as float64.
+
 template <typename R, int N, int NL, int T>
-__global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src, cplx<R>* __restrict__ keep,
-                                                 long long P, int W, int last, int axis, int fixk, int nchunk,
-                                                 int flags, int kmax_par, int kmax_per, int n1, int n3,
-                                                 const double* __restrict__ wing, const cplx<R>* __restrict__ twg,
-                                                 double* __restrict__ part, int RS) {
+__device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
+    const cplx<R>* __restrict__ src = A.src;
+    cplx<R>* __restrict__ keep = A.keep;
+    const long long P = A.P;
+    const int W = A.W, last = A.last, axis = A.axis, fixk = A.fixk, nchunk = A.nchunk, flags = A.flags;
+    const int kmax_par = A.kmax_par, kmax_per = A.kmax_per, n1 = A.n1, n3 = A.n3;
+    const double* __restrict__ wing = A.win;
+    const cplx<R>* __restrict__ twg = A.tw;
+    double* __restrict__ part = A.part;
+    const int RS = A.RS;
     constexpr int TPC = T / NL;              // threads per column
     constexpr int U = N / 2 + 1;             // row pairs (k, N-k), k = 0 .. N/2 (k = 0, N/2 alone)
     constexpr int EU = (U + TPC - 1) / TPC;  // consecutive pairs per thread in the epilogue
@@ -403,15 +409,15 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     // column blocks of this block: items it = 0 .. nit-1
     int u0, kzb0, nit;
     if (fixk) {
-        const int kzb = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+        const int kzb = B.x / nchunk, ch = B.x % nchunk;
         const int UC = (N + nchunk - 1) / nchunk;
         u0 = ch * UC;
         nit = max(0, min(N, u0 + UC) - u0);
         kzb0 = kzb;
     } else {
         const int KB = (nkzb + nchunk - 1) / nchunk;
-        u0 = blockIdx.x / nchunk;
-        kzb0 = (blockIdx.x % nchunk) * KB;
+        u0 = B.x / nchunk;
+        kzb0 = (B.x % nchunk) * KB;
         nit = max(0, min(nkzb, kzb0 + KB) - kzb0);
     }
     auto item = [&](int it, int& u, int& kzz) {
@@ -567,7 +573,7 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
     }
     // this block's histograms -> its row of the partials (summed by k_bin_reduce31 / k_bin_reduce2)
     __syncthreads();
-    double* row = part + (long long)blockIdx.x * RS;
+    double* row = part + (long long)B.x * RS;
     const int n31 = 5 * n3 + 2 * n1;
     for (int i = tid; i < 4 * n3 + n1; i += T) row[i] = hs[i];
     for (int i = tid; i < n3 + n1; i += T) row[4 * n3 + n1 + i] = (double)c3[i];
@@ -576,14 +582,29 @@ __global__ __launch_bounds__(T) void k_last_bin(const cplx<R>* __restrict__ src,
         row[n31 + n2w + i] = (double)c2[i];
     }
 }
+template <typename R, int N, int NL, int T>
+__global__ __launch_bounds__(T) void k_last_bin(LastBinArgs<R> a) {
+    last_bin_body<R, N, NL, T>(a, own_blk());
+}
+template <typename R, int N, int NL, int T>
+__global__ __launch_bounds__(T) void k_last_bin_b(const LastBinArgs<R>* __restrict__ tab,
+                                                   const unsigned* __restrict__ start, int nj) {
+    PYL_BATCH_BLK(start, nj);
+    last_bin_body<R, N, NL, T>(tab[bj_], blk_);
+}
 
 // Sum the per-block histogram rows of k_last_bin into the pass sums.
 // 3-D and 1-D entries (k_bin_reduce31): 64 entries per workgroup, the blocks split over the 4 waves
 // and over gridDim.y slices; each slice adds its total with one atomic.  Entries: 3-D sums, 1-D
 // sums, 3-D counts, 1-D counts.
-__global__ __launch_bounds__(256) void k_bin_reduce31(const double* __restrict__ part, int RS, int nblocks, int n3,
-                                                      int n1, double* __restrict__ g3, double* __restrict__ g1,
-                                                      double* __restrict__ gc3, double* __restrict__ gc1) {
+
+__device__ __forceinline__ void reduce31_body(const ReduceArgs& A) {
+    const double* __restrict__ part = A.part;
+    const int RS = A.RS, nblocks = A.nblocks, n3 = A.n3, n1 = A.n1;
+    double* __restrict__ g3 = A.g3;
+    double* __restrict__ g1 = A.g1;
+    double* __restrict__ gc3 = A.gc3;
+    double* __restrict__ gc1 = A.gc1;
     __shared__ double red[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -605,14 +626,23 @@ __global__ __launch_bounds__(256) void k_bin_reduce31(const double* __restrict__
         if (t != 0.0) atomicAdd(dst, t);
     }
 }
+__global__ __launch_bounds__(256) void k_bin_reduce31(ReduceArgs a) { reduce31_body(a); }
+// batched: job blockIdx.z
+__global__ __launch_bounds__(256) void k_bin_reduce31_b(const ReduceArgs* __restrict__ tab) {
+    const ReduceArgs& a = tab[blockIdx.z];
+    reduce31_body(a);
+}
 
 // 2-D entries (k_bin_reduce2): (k_per, k_par) comes from the blocks whose window holds k_par:
 // fixk = 0, the blocks of u = k_par and u = N - k_par (k_par = |k_u|); fixk = 1, the blocks of kz
 // block k_par / NL, row k_par % NL.
 // The 2-D counts sit n2w entries after the 2-D sums of a row.
-__global__ void k_bin_reduce2(const double* __restrict__ part, int RS, int nblocks, int n31, int n2w, int nper,
-                              int kmax_par, int fixk, int nchunk, int NL, int N, double* __restrict__ g2,
-                              double* __restrict__ gc2) {
+__device__ __forceinline__ void reduce2_body(const ReduceArgs& A) {
+    const double* __restrict__ part = A.part;
+    const int RS = A.RS, nblocks = A.nblocks, n31 = 5 * A.n3 + 2 * A.n1, n2w = A.n2w, nper = A.nper;
+    const int kmax_par = A.kmax_par, fixk = A.fixk, nchunk = A.nchunk, NL = A.NL, N = A.N;
+    double* __restrict__ g2 = A.g2;
+    double* __restrict__ gc2 = A.gc2;
     const long long i2 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i2 >= (long long)(kmax_par + 1) * nper) return;
     const int kpar = (int)(i2 % (kmax_par + 1)), kper = (int)(i2 / (kmax_par + 1));
@@ -639,6 +669,12 @@ __global__ void k_bin_reduce2(const double* __restrict__ part, int RS, int nbloc
     if (a != 0.0) g2[i2] += a;
     if (m != 0.0) gc2[i2] += m;
 }
+__global__ void k_bin_reduce2(ReduceArgs a) { reduce2_body(a); }
+// batched: job blockIdx.y
+__global__ void k_bin_reduce2_b(const ReduceArgs* __restrict__ tab) {
+    const ReduceArgs& a = tab[blockIdx.y];
+    reduce2_body(a);
+}
 
 static int num_cus() {
     static int n = 0;
@@ -653,49 +689,88 @@ static int num_cus() {
 
 int pk_last_axis(int axis) { return axis == 0 ? 1 : 0; }
 
+// launch shape of the binning pass for one grid size and line-of-sight class: blocks own one kz block
+// (fixk, line of sight z) or one u; about two blocks per CU, all resident at once, each walking its
+// column blocks with prefetch (four per CU with fixk measured faster)
+template <typename R, int N, bool FIXK> struct BinShape {
+    static constexpr int NL = nl_bin<R, N, FIXK>();
+    static constexpr int T = threads_bin<N, NL>();
+    static constexpr int W = N / 2 + 1;   // kz = 0 .. N/2; rows of the spectrum may be padded to P >= W
+    int nchunk, grid, n2w, RS, n31;
+    long long n2;
+    size_t shm;
+    PkDims d;
+    BinShape() {
+        d = pk_dims(N, W);
+        const int nkzb = (W + NL - 1) / NL;
+        const int want = (FIXK ? 4 : 2) * num_cus();
+        if (FIXK) {
+            nchunk = std::min(N, std::max(1, want / nkzb));
+            grid = nkzb * nchunk;
+        } else {
+            nchunk = std::min(nkzb, std::max(1, (want + N - 1) / N));   // at least `want` blocks when it can
+            grid = N * nchunk;
+        }
+        const int nper = d.kmax_per + 1;
+        n2w = (FIXK ? NL : 1) * nper;
+        RS = 5 * d.n3() + 2 * d.n1() + 2 * n2w;
+        n31 = 5 * d.n3() + 2 * d.n1();
+        n2 = (long long)(d.kmax_par + 1) * nper;
+        // LDS: float64 sums (4 n3 + n1 + n2w), then 32-bit counts (n3 + n1 + n2w)
+        shm = sizeof(double) * (size_t)(4 * d.n3() + d.n1() + n2w) + sizeof(unsigned) * (size_t)(d.n3() + d.n1() + n2w);
+    }
+    int slices() const { return std::max(1, std::min(16, grid / 32)); }
+};
+
 template <typename R, int N, bool FIXK>
-static int launch_last_bin_k(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
-                             const PkSums& sums, hipStream_t st) {
-    constexpr int NL = nl_bin<R, N, FIXK>();
-    constexpr int T = threads_bin<N, NL>();
-    const int W = N / 2 + 1;   // kz = 0 .. N/2; rows of the spectrum may be padded to P >= W
-    const PkDims d = pk_dims(N, W);
+static int last_bin_job_k(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
+                          const PkSums& sums, double* part, LastBinArgs<R>* la, ReduceArgs* ra, unsigned* blocks) {
+    const BinShape<R, N, FIXK> sh;
     const cplx<R>* tw = twiddles<R>(N);
     const double* win = window_table(N, p);
     PYL_CHECK(tw && win, E_ALLOC, "table allocation failed");
-    const int last = pk_last_axis(axis);
+    const PkDims& d = sh.d;
     const int fixk = FIXK ? 1 : 0;
-    const int nkzb = (W + NL - 1) / NL;
-    // about two blocks per CU, all resident at once, each walking its column blocks with prefetch;
-    // for the line of sight along z (kz blocks split over u chunks) four per CU measured faster
-    const int want = (fixk ? 4 : 2) * num_cus();
-    int nchunk, grid;
-    if (fixk) {
-        nchunk = std::min(N, std::max(1, want / nkzb));
-        grid = nkzb * nchunk;
-    } else {
-        nchunk = std::min(nkzb, std::max(1, (want + N - 1) / N));   // at least `want` blocks when it can
-        grid = N * nchunk;
-    }
-    const int nper = d.kmax_per + 1;
-    const int n2w = (fixk ? NL : 1) * nper;
-    const int RS = 5 * d.n3() + 2 * d.n1() + 2 * n2w;
-    // LDS: float64 sums (4 n3 + n1 + n2w), then 32-bit counts (n3 + n1 + n2w)
-    size_t shm = sizeof(double) * (size_t)(4 * d.n3() + d.n1() + n2w) + sizeof(unsigned) * (size_t)(d.n3() + d.n1() + n2w);
+    *la = LastBinArgs<R>{spec, keep, (long long)P, sh.W, pk_last_axis(axis), axis, fixk, sh.nchunk, flags,
+                         d.kmax_par, d.kmax_per, d.n1(), d.n3(), win, tw, part, sh.RS};
+    *ra = ReduceArgs{part, sh.RS, sh.grid, d.n3(), d.n1(), sh.n2w, d.kmax_per + 1, d.kmax_par, fixk, sh.nchunk,
+                     sh.NL, N, sums.s3, sums.s1, sums.c3, sums.c1, sums.s2, sums.c2};
+    *blocks = (unsigned)sh.grid;
+    return OK;
+}
+
+template <typename R, int N, bool FIXK>
+static int launch_last_bin_k(const cplx<R>* spec, cplx<R>* keep, int64_t P, int axis, int p, int flags,
+                             const PkSums& sums, hipStream_t st) {
+    const BinShape<R, N, FIXK> sh;
     int rc = OK;
-    double* part = (double*)workspace(WS_PART, sizeof(double) * (size_t)grid * RS, &rc);
+    double* part = (double*)workspace(WS_PART, sizeof(double) * (size_t)sh.grid * sh.RS, &rc);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_last_bin<R, N, NL, T>), dim3((unsigned)grid), dim3(T), shm, st, spec, keep, (long long)P, W,
-                       last, axis, fixk, nchunk, flags, d.kmax_par, d.kmax_per, d.n1(), d.n3(), win, tw, part, RS);
+    LastBinArgs<R> la;
+    ReduceArgs ra;
+    unsigned grid = 0;
+    rc = last_bin_job_k<R, N, FIXK>(spec, keep, P, axis, p, flags, sums, part, &la, &ra, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_last_bin<R, N, sh.NL, sh.T>), dim3(grid), dim3(sh.T), sh.shm, st, la);
     PYL_HIP(hipGetLastError());
-    const int n31 = 5 * d.n3() + 2 * d.n1();
-    const int slices = std::max(1, std::min(16, grid / 32));
-    hipLaunchKernelGGL(k_bin_reduce31, dim3((unsigned)((n31 + 63) / 64), (unsigned)slices), dim3(256), 0, st, part, RS,
-                       grid, d.n3(), d.n1(), sums.s3, sums.s1, sums.c3, sums.c1);
+    hipLaunchKernelGGL(k_bin_reduce31, dim3((unsigned)((sh.n31 + 63) / 64), (unsigned)sh.slices()), dim3(256), 0, st,
+                       ra);
     PYL_HIP(hipGetLastError());
-    const long long n2 = (long long)(d.kmax_par + 1) * nper;
-    hipLaunchKernelGGL(k_bin_reduce2, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, part, RS, grid, n31, n2w,
-                       nper, d.kmax_par, fixk, nchunk, NL, N, sums.s2, sums.c2);
+    hipLaunchKernelGGL(k_bin_reduce2, dim3((unsigned)((sh.n2 + 255) / 256)), dim3(256), 0, st, ra);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R, int N, bool FIXK>
+static int launch_last_bin_batch_k(const LastBinArgs<R>* tab, const unsigned* start, int nj, unsigned total,
+                                   const ReduceArgs* rtab, hipStream_t st) {
+    const BinShape<R, N, FIXK> sh;
+    hipLaunchKernelGGL((k_last_bin_b<R, N, sh.NL, sh.T>), dim3(total), dim3(sh.T), sh.shm, st, tab, start, nj);
+    PYL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bin_reduce31_b, dim3((unsigned)((sh.n31 + 63) / 64), (unsigned)sh.slices(), (unsigned)nj),
+                       dim3(256), 0, st, rtab);
+    PYL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bin_reduce2_b, dim3((unsigned)((sh.n2 + 255) / 256), (unsigned)nj), dim3(256), 0, st, rtab);
     PYL_HIP(hipGetLastError());
     return OK;
 }
@@ -731,6 +806,62 @@ template int pk_last_pass_bin<float>(const cplx<float>*, cplx<float>*, int, int6
 template int pk_last_pass_bin<double>(const cplx<double>*, cplx<double>*, int, int64_t, int, int, int, const PkSums&,
                                       hipStream_t);
 
+size_t last_bin_part_doubles(int N, int axis, int prec) {
+    switch (N) {
+#define PYL_CASE(n)                                                                                          \
+    case n:                                                                                                  \
+        if (prec == 8)                                                                                       \
+            return axis == 2 ? (size_t)BinShape<double, n, true>().grid * BinShape<double, n, true>().RS     \
+                             : (size_t)BinShape<double, n, false>().grid * BinShape<double, n, false>().RS;  \
+        return axis == 2 ? (size_t)BinShape<float, n, true>().grid * BinShape<float, n, true>().RS           \
+                         : (size_t)BinShape<float, n, false>().grid * BinShape<float, n, false>().RS;
+        PYL_PK_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            return 0;
+    }
+}
+
+template <typename R>
+int last_bin_job(const cplx<R>* spec, int N, int64_t P, int axis, int mas_p, int flags, const PkSums& sums,
+                 double* part, LastBinArgs<R>* a, ReduceArgs* r, unsigned* blocks) {
+    switch (N) {
+#define PYL_CASE(n)                                                                                      \
+    case n:                                                                                              \
+        return axis == 2 ? last_bin_job_k<R, n, true>(spec, nullptr, P, axis, mas_p, flags, sums, part, a, r, blocks) \
+                         : last_bin_job_k<R, n, false>(spec, nullptr, P, axis, mas_p, flags, sums, part, a, r, blocks);
+        PYL_PK_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("grid size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+
+template <typename R>
+int last_bin_batch(int N, int axis, const LastBinArgs<R>* tab, const unsigned* start, int nj, unsigned total,
+                   const ReduceArgs* rtab, hipStream_t st) {
+    switch (N) {
+#define PYL_CASE(n)                                                                                      \
+    case n:                                                                                              \
+        return axis == 2 ? launch_last_bin_batch_k<R, n, true>(tab, start, nj, total, rtab, st)         \
+                         : launch_last_bin_batch_k<R, n, false>(tab, start, nj, total, rtab, st);
+        PYL_PK_SIZES(PYL_CASE)
+#undef PYL_CASE
+        default:
+            set_error("grid size %d not supported", N);
+            return E_UNSUPPORTED;
+    }
+}
+template int last_bin_job<float>(const cplx<float>*, int, int64_t, int, int, int, const PkSums&, double*,
+                                 LastBinArgs<float>*, ReduceArgs*, unsigned*);
+template int last_bin_job<double>(const cplx<double>*, int, int64_t, int, int, int, const PkSums&, double*,
+                                  LastBinArgs<double>*, ReduceArgs*, unsigned*);
+template int last_bin_batch<float>(int, int, const LastBinArgs<float>*, const unsigned*, int, unsigned,
+                                   const ReduceArgs*, hipStream_t);
+template int last_bin_batch<double>(int, int, const LastBinArgs<double>*, const unsigned*, int, unsigned,
+                                    const ReduceArgs*, hipStream_t);
+
 // ------------------------------------------------------------------ finalize
 size_t pk_sums_doubles(const PkDims& d) { return 5 * (size_t)d.n3() + 2 * (size_t)d.n1() + 2 * (size_t)d.n2() + 1; }
 
@@ -748,16 +879,25 @@ void pk_sums_carve(const PkDims& d, double* b, PkSums* s) {
 // compared, bin by bin, with the geometry (the 3-D comparison includes the DC bin, so it contains
 // the total test of check_number_modes, Pk_library.pyx:90-102); histograms that were not binned
 // report the geometry.
-__global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxSize, int flags,
-                              const unsigned long long* __restrict__ n1c, const unsigned long long* __restrict__ n2c,
-                              const unsigned long long* __restrict__ n3c, const double* __restrict__ k3s,
-                              const double* __restrict__ s3, const double* __restrict__ s1,
-                              const double* __restrict__ s2, const double* __restrict__ c3,
-                              const double* __restrict__ c1, const double* __restrict__ c2,
-                              unsigned* __restrict__ status, double* __restrict__ out) {
+
+__device__ __forceinline__ void finalize_body(const FinArgs& A, Blk B) {
     // evaluate as written (no fused multiply-add), as the reference's Python arithmetic does:
     // the last 1-D bin needs kN^2 - k^2 == 0 exactly
 #pragma clang fp contract(off)
+    const int N = A.N, kmax = A.kmax, kmax_par = A.kmax_par, n2 = A.n2, flags = A.flags;
+    const double BoxSize = A.BoxSize;
+    const unsigned long long* __restrict__ n1c = A.n1c;
+    const unsigned long long* __restrict__ n2c = A.n2c;
+    const unsigned long long* __restrict__ n3c = A.n3c;
+    const double* __restrict__ k3s = A.k3s;
+    const double* __restrict__ s3 = A.s3;
+    const double* __restrict__ s1 = A.s1;
+    const double* __restrict__ s2 = A.s2;
+    const double* __restrict__ c3 = A.c3;
+    const double* __restrict__ c1 = A.c1;
+    const double* __restrict__ c2 = A.c2;
+    unsigned* __restrict__ status = A.status;
+    double* __restrict__ out = A.out;
     const double kF = 2.0 * M_PI / BoxSize;
     const int middle = N / 2;
     const double kN = middle * kF;
@@ -776,7 +916,7 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
     double* P2D = kper + n2;
     double* N2 = P2D + n2;
     const bool do1 = flags & PK_1D, do2 = flags & PK_2D;
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    const int gi = B.x * blockDim.x + threadIdx.x, gs = B.gx * blockDim.x;
     unsigned bad = 0u;
     for (int i = gi; i <= kmax; i += gs) {
         const double n = c3[i];
@@ -812,13 +952,33 @@ __global__ void k_pk_finalize(int N, int kmax, int kmax_par, int n2, double BoxS
     }
     if (bad) atomicAdd(status, bad);
 }
+__global__ void k_pk_finalize(FinArgs a) { finalize_body(a, own_blk()); }
+// batched: job blockIdx.y
+__global__ void k_pk_finalize_b(const FinArgs* __restrict__ tab) {
+    const FinArgs& a = tab[blockIdx.y];
+    finalize_body(a, own_blk());
+}
+
+FinArgs finalize_args(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out) {
+    const PkDims& d = g->d;
+    return FinArgs{d.N, d.kmax, d.kmax_par, d.n2(), BoxSize, flags, g->n1, g->n2, g->n3, g->k3sum, sums.s3, sums.s1,
+                   sums.s2, sums.c3, sums.c1, sums.c2, sums.status, out};
+}
+
+static unsigned finalize_blocks(int N) {
+    const PkDims d = pk_dims(N, half_len(N));
+    return (unsigned)((std::max(d.kmax + 1, d.n2()) + 255) / 256);
+}
+
+int finalize_batch(int N, const FinArgs* tab, int nj, hipStream_t st) {
+    hipLaunchKernelGGL(k_pk_finalize_b, dim3(finalize_blocks(N), (unsigned)nj), dim3(256), 0, st, tab);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
 
 int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out, hipStream_t st) {
-    const PkDims& d = g->d;
-    const int m = std::max(d.kmax + 1, d.n2());
-    hipLaunchKernelGGL(k_pk_finalize, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, d.N, d.kmax, d.kmax_par,
-                       d.n2(), BoxSize, flags, g->n1, g->n2, g->n3, g->k3sum, sums.s3, sums.s1, sums.s2, sums.c3,
-                       sums.c1, sums.c2, sums.status, out);
+    const FinArgs fa = finalize_args(g, sums, BoxSize, flags, out);
+    hipLaunchKernelGGL(k_pk_finalize, dim3(finalize_blocks(g->d.N)), dim3(256), 0, st, fa);
     PYL_HIP(hipGetLastError());
     return OK;
 }

@@ -67,6 +67,61 @@ int pk_last_pass_bin(const cplx<R>* spec, cplx<R>* keep, int N, int64_t P, int a
 size_t pk_out_len(const PkDims& d);
 int pk_finalize(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out, hipStream_t st);
 
+// Argument blocks of the fused last pass + binning, of the two reductions of its per-block
+// histograms and of the normalisation (shared with the batch planner)
+template <typename R> struct LastBinArgs {
+    const cplx<R>* src;
+    cplx<R>* keep;
+    long long P;
+    int W, last, axis, fixk, nchunk, flags, kmax_par, kmax_per, n1, n3;
+    const double* win;
+    const cplx<R>* tw;
+    double* part;
+    int RS;
+};
+
+struct ReduceArgs {
+    const double* part;
+    int RS, nblocks, n3, n1, n2w, nper, kmax_par, fixk, nchunk, NL, N;
+    double *g3, *g1, *gc3, *gc1, *g2, *gc2;
+};
+
+struct FinArgs {
+    int N, kmax, kmax_par, n2;
+    double BoxSize;
+    int flags;
+    const unsigned long long *n1c, *n2c, *n3c;
+    const double* k3s;
+    const double *s3, *s1, *s2, *c3, *c1, *c2;
+    unsigned* status;
+    double* out;
+};
+
+// Batched last pass + binning + reductions + normalisation over jobs of one grid size N and line
+// of sight class (z or not): *_job builds a job's arguments (part: its per-block histogram rows,
+// last_bin_part_doubles(N, axis, prec) of them) and block count; last_bin_batch launches the
+// tables (block ranges start[0..nj]).
+size_t last_bin_part_doubles(int N, int axis, int prec);
+template <typename R>
+int last_bin_job(const cplx<R>* spec, int N, int64_t P, int axis, int mas_p, int flags, const PkSums& sums,
+                 double* part, LastBinArgs<R>* a, ReduceArgs* r, unsigned* blocks);
+template <typename R>
+int last_bin_batch(int N, int axis, const LastBinArgs<R>* tab, const unsigned* start, int nj, unsigned total,
+                   const ReduceArgs* rtab, hipStream_t st);
+FinArgs finalize_args(const PkGeometry* g, const PkSums& sums, double BoxSize, int flags, double* out);
+int finalize_batch(int N, const FinArgs* tab, int nj, hipStream_t st);
+
+// Deposit algorithm of the fused particles -> P(k) path: cell-binned, direct atomic, or cell-binned
+// after a presort by supertile.  deposit_kind: the one `flags` forces (DEP_ATOMIC / DEP_BINNED /
+// DEP_SORTED), else the autotuner's (timing trials the first time a size class is seen, which
+// overwrite `grid`).  pk_particles: the whole fused path of one job (pyl_pk_particles).
+enum DepositKind { DK_BINNED = 0, DK_ATOMIC = 1, DK_SORTED = 2 };
+struct Rsd;
+int deposit_kind(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, bool f64,
+                 int flags, void* grid, const double* dmass, hipStream_t st, int* kind, const Rsd& rsd);
+int pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme, int axis,
+                 int prec, int flags, double* out, hipStream_t st, const Rsd& rsd);
+
 // The whole estimator from a real (N,N,N) field: pass Z r2c (affine: delta = rho * N^3 / mass - 1
 // applied on the load, mass = dmass[0] when dmass is non-null), the column pass, the fused last pass
 // + binning, the normalisation and the mode check (see pyl_pk).  keep (nullable) receives the

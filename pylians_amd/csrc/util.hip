@@ -21,11 +21,21 @@ const char* last_error() { return g_err; }
 
 // Zero-fill with a kernel rather than hipMemsetAsync, so that the clear is an ordinary kernel
 // node when a caller captures the stream into a graph.
-__global__ void k_zero(uint4* __restrict__ p, long long n16, unsigned char* __restrict__ tail, int ntail) {
+__device__ __forceinline__ void zero_body(uint4* __restrict__ p, long long n16, unsigned char* __restrict__ tail,
+                                          int ntail) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
         p[i] = make_uint4(0u, 0u, 0u, 0u);
     if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+__global__ void k_zero(uint4* __restrict__ p, long long n16, unsigned char* __restrict__ tail, int ntail) {
+    zero_body(p, n16, tail, ntail);
+}
+// batched: buffer blockIdx.y of the table (16-byte aligned pointer, byte count)
+__global__ void k_zero_b(const ZeroArgs* __restrict__ tab) {
+    const ZeroArgs z = tab[blockIdx.y];
+    const long long n16 = (long long)(z.bytes / 16);
+    zero_body((uint4*)z.ptr, n16, (unsigned char*)z.ptr + n16 * 16, (int)(z.bytes % 16));
 }
 
 int zero_fill(void* ptr, size_t bytes, hipStream_t st) {

@@ -5,6 +5,12 @@
 namespace pyl {
 
 int zero_fill(void* ptr, size_t bytes, hipStream_t st);
+// one entry of a batched zero-fill (k_zero_b: buffer blockIdx.y)
+struct ZeroArgs {
+    void* ptr;
+    size_t bytes;
+};
+__global__ void k_zero_b(const ZeroArgs* __restrict__ tab);
 
 // out[0] = float64 sum of part[0 .. nb) by one workgroup (the second step of per-workgroup
 // partial sums, in place of one same-address atomic per workgroup)
@@ -14,7 +20,7 @@ int sum_parts(const double* part, int nb, double* out, hipStream_t st);
 enum WsSlot { WS_SPEC = 0, WS_SPEC2 = 1, WS_SUMS = 2, WS_GRID = 3, WS_BINS = 4, WS_AUX = 5, WS_AUX2 = 6,
               WS_AUX3 = 7, WS_HOST_POS = 8, WS_HOST_GRID = 9, WS_HOST_W = 10, WS_OUT = 11, WS_MASS = 12,
               WS_PART = 13, WS_STAGE = 14, WS_SUP = 15, WS_C2R = 16,
-              WS_RED = 17 };
+              WS_RED = 17, WS_BATCH = 18, WS_BATCH_TABLES = 19 };
 void* workspace(int slot, size_t bytes, int* rc);
 
 // A private set of workspaces (one buffer per slot) owned by a captured hipGraph.  While an

@@ -9,6 +9,11 @@ from tests._gpuref import assert_pk_close, oracle_pk_particles
 
 pytestmark = pytest.mark.gpu
 
+# Replays and eager runs differ only in the order of the float64 LDS additions of the cell sums: a
+# cell whose exact sum lies within float64 rounding of a float32 rounding boundary (about one in 10^9)
+# can round the other way, which moves a bin by up to ~1e-9 relative on these grids.
+RT = 1e-8
+
 
 def _jobs():
     rng = np.random.default_rng(100)
@@ -34,8 +39,8 @@ def test_graph_batch_matches_eager_and_oracle(gpu):
         assert float((x * 0).sum()) == 0.0
         for g, e in zip(res, eager):
             assert np.array_equal(g.Nmodes3D, e.Nmodes3D)
-            assert np.allclose(g.Pk, e.Pk, rtol=1e-12, atol=0)
-            assert np.allclose(g.Pk2D, e.Pk2D, rtol=1e-12, atol=1e-300)
+            assert np.allclose(g.Pk, e.Pk, rtol=RT, atol=0)
+            assert np.allclose(g.Pk2D, e.Pk2D, rtol=RT, atol=1e-300)
     for g, j in zip(res, jobs):
         o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_noise=True)
         assert_pk_close(g, o, rtol=1e-5, noise=fl)
@@ -83,7 +88,7 @@ def test_graph_batch_presorted_weighted(gpu):
         assert float((x * 0).sum()) == 0.0
         for g, e in zip(res, eager):
             assert np.array_equal(g.Nmodes3D, e.Nmodes3D)
-            assert np.allclose(g.Pk, e.Pk, rtol=1e-11, atol=0)
+            assert np.allclose(g.Pk, e.Pk, rtol=RT, atol=0)
     for g, j in zip(res, jobs):
         o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], W=j["W"], with_noise=True)
         assert_pk_close(g, o, rtol=1e-5, noise=fl)
@@ -109,8 +114,8 @@ def test_graph_owns_its_workspaces(gpu):
         res = b.results()
         for g, e in zip(res, eager):
             assert np.array_equal(g.Nmodes3D, e.Nmodes3D)
-            assert np.allclose(g.Pk, e.Pk, rtol=1e-12, atol=0)
-            assert np.allclose(g.Pk1D, e.Pk1D, rtol=1e-12, atol=1e-300)
+            assert np.allclose(g.Pk, e.Pk, rtol=RT, atol=0)
+            assert np.allclose(g.Pk1D, e.Pk1D, rtol=RT, atol=1e-300)
         assert int(big.Nmodes3D.sum()) + 1 == (dims ** 3 - 8) // 2 + 8
     for g, j in zip(res, jobs):
         o, fl = oracle_pk_particles(j["pos"], j["BoxSize"], j["dims"], "CIC", j["axis"], with_noise=True)
