@@ -31,6 +31,10 @@ constexpr int BT = 512;    // k_bin: threads per workgroup
 constexpr int KP = 3;      // k_bin: particles per thread per step
 constexpr int HT = 2048;   // k_bin: LDS hash slots (> BT * KP, so an insertion always finds a slot)
 constexpr int TT = 512;    // k_tile: threads per workgroup
+// row stride of the float64 LDS accumulator of a tile: TILE + 1 doubles, so that particles one row
+// apart in y (neighbours in the input along z fill a row, then the next) land on LDS banks 34 apart
+// (measured on the 512^3 lattice: k_tile 0.88 -> 0.77 ms; strides 18/20/24 and padded planes were slower)
+constexpr int TROW = TILE + 1;
 static_assert(HT % BT == 0, "hash slots per thread");
 static_assert(HT > BT * KP, "every key of a step must find a hash slot");
 
@@ -378,7 +382,7 @@ __device__ __forceinline__ void accumulate(double* acc, const float* r, int N, i
             const int c1 = l1 + m;
             if (CLIP && (c1 < 0 || c1 >= TILE)) continue;
             const double wlm = UNIT ? w0[l] * w1[m] : w0[l] * w1[m] * ww;   // UNIT: no weights, mult == 1
-            double* row = acc + (c0 * TILE + c1) * TILE;
+            double* row = acc + (c0 * TILE + c1) * TROW;
 #pragma unroll
             for (int q = 0; q < S; ++q) {
                 const int c2 = l2 + q;
@@ -410,7 +414,7 @@ __device__ __forceinline__ void accumulate32(double* acc, const float* r, int N,
             const int c1 = l1 + m;
             if (CLIP && (c1 < 0 || c1 >= TILE)) continue;
             const float wlm = UNIT ? w0[l] * w1[m] : w0[l] * w1[m] * ww;
-            double* row = acc + (c0 * TILE + c1) * TILE;
+            double* row = acc + (c0 * TILE + c1) * TROW;
 #pragma unroll
             for (int q = 0; q < S; ++q) {
                 const int c2 = l2 + q;
@@ -431,7 +435,7 @@ template <int S, typename G, bool HASW, int MODE, bool UNIT, int ACC>
 __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
     constexpr int R = HASW ? 4 : 3;
     constexpr int NE = 72;   // 8 neighbours (d = 0 own) x (8 class segments + 1 overflow group)
-    __shared__ double acc[TILE * TILE * TILE];
+    __shared__ double acc[TILE * TILE * TROW];
     __shared__ long long sbase[NE];
     __shared__ unsigned scnt[NE];
     __shared__ unsigned sstart[NE + 1];
@@ -442,7 +446,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
     const int t = (int)(B.x % XCDS) * tper + (int)(B.x / XCDS);
     if (t >= ntl) return;
     const int tx = t / (nt * nt), ty = (t / nt) % nt, tz = t % nt;
-    for (int i = tid; i < TILE * TILE * TILE; i += TT) acc[i] = 0.0;
+    for (int i = tid; i < TILE * TILE * TROW; i += TT) acc[i] = 0.0;
     // entry e = d * 9 + c: class segment c (< 8) or the overflow group (c = 8) of the neighbour
     // t - d; a neighbour's particles reach this tile when their class contains d
     if (tid < NE) {
@@ -516,9 +520,41 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
             cg[j] = ng[j];
         }
     };
-    fetch(tid);
+    // phase 1: the tile's own class-0 records (most of them): one contiguous segment whose stencils
+    // stay inside the tile -- no entry search, no clipping, no overflow class word
+    const unsigned n0 = scnt[0];
+    {
+        const float* q0 = a.bucket + sbase[0] * R;
+        auto fetch0 = [&](unsigned f) {
+#pragma unroll
+            for (int j = 0; j < RP; ++j) {
+                const unsigned fj = f + j * TT;
+                const float* q = q0 + (size_t)(fj < n0 ? fj : 0u) * R;   // record 0 of the segment always exists
+#pragma unroll
+                for (int c = 0; c < R; ++c) nr[j][c] = q[c];
+            }
+        };
+        fetch0(tid);
+        for (unsigned f = tid; f < n0; f += RP * TT) {
+#pragma unroll
+            for (int j = 0; j < RP; ++j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) cr[j][c] = nr[j][c];
+            fetch0(f + RP * TT);
+#pragma unroll
+            for (int j = 0; j < RP; ++j) {
+                if (f + j * TT >= n0) break;
+                if (ACC == ACC_F32W)
+                    accumulate32<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, (float)a.mult);
+                else
+                    accumulate<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
+            }
+        }
+    }
+    // phase 2: halo-class segments of this tile and its lower neighbours, overflow groups
+    fetch(n0 + tid);
     advance();
-    for (unsigned f = tid; f < tot; f += RP * TT) {
+    for (unsigned f = n0 + tid; f < tot; f += RP * TT) {
         fetch(f + RP * TT);
 #pragma unroll
         for (int j = 0; j < RP; ++j) {
@@ -528,14 +564,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
             const unsigned d = (unsigned)(ej / 9);
             const bool ok = ej >= 0 && (ej % 9 < 8 || (cg[j] & d) == d);
             if (ACC == ACC_F32W) {
-                if (ej == 0)
-                    accumulate32<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, (float)a.mult);
-                else if (ok)
-                    accumulate32<S, HASW, true, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, (float)a.mult);
-            } else if (ej == 0)
-                accumulate<S, HASW, false, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
-            else if (ok)
+                if (ok) accumulate32<S, HASW, true, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, (float)a.mult);
+            } else if (ok) {
                 accumulate<S, HASW, true, UNIT>(acc, cr[j], a.N, ox, oy, oz, a.inv, a.mult);
+            }
         }
         advance();
     }
@@ -546,7 +578,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
     for (int idx = tid; idx < TILE * TILE * TILE; idx += TT) {
         const int i = idx / (TILE * TILE), j = (idx / TILE) % TILE, k = idx % TILE;
         const long long g = (((long long)ox + i) * N + oy + j) * N + oz + k;
-        put<G, MODE>(grid, g, acc[idx], am);
+        put<G, MODE>(grid, g, acc[(i * TILE + j) * TROW + k], am);
     }
 }
 
