@@ -2,6 +2,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -233,8 +234,8 @@ extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int
 // ------------------------------------------------------------------ deposit autotuner
 // Per (device, dims, scheme, precision, weights, floor(log2 n)) the fused pipeline times the
 // cell-binned deposit, the direct-atomic deposit and the binned deposit after a presort by
-// supertile once (all overwrite the grid, so the trial runs are harmless), keeps the fastest and
-// caches the choice.  pyl_tune_export / pyl_tune_import move the cache as text ("dims scheme prec
+// supertile once on a prefix of the particles (all overwrite the grid, so the trial runs are
+// harmless), keeps the fastest and caches the choice.  pyl_tune_export / pyl_tune_import move the cache as text ("dims scheme prec
 // hasw nlog choice" lines; choice 0 = binned, 1 = atomic, 2 = presorted binned).
 namespace {
 struct TuneKey {
@@ -301,12 +302,14 @@ static int tune_choice(const float* pos, const float* W, int64_t n, int dims, do
     for (auto& x : e) PYL_HIP(hipEventCreate(&x));
     float ms[3] = {0.f, 0.f, 0.f};
     int rc = OK;
+    // the trials deposit a prefix of the particles (a quarter of them, at least 2^22): the same
+    // density of particles per tile for unordered inputs, a slab of the box for ordered ones
+    const int64_t ns = n > (1LL << 22) ? std::max<int64_t>(1LL << 22, n / 4) : n;
     for (int c = 0; c < 3 && rc == OK; ++c) {
-        rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);  // warm
+        rc = deposit_delta(pos, W, ns, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);  // warm
         if (rc) break;
         PYL_HIP(hipEventRecord(e[0], st));
-        for (int r = 0; r < 2 && rc == OK; ++r)
-            rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);
+        rc = deposit_delta(pos, W, ns, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);
         PYL_HIP(hipEventRecord(e[1], st));
         PYL_HIP(hipEventSynchronize(e[1]));
         PYL_HIP(hipEventElapsedTime(&ms[c], e[0], e[1]));
