@@ -234,7 +234,7 @@ extern "C" int pyl_pk(const void* delta, int dims, double BoxSize, int axis, int
 // ------------------------------------------------------------------ deposit autotuner
 // Per (device, dims, scheme, precision, weights, floor(log2 n)) the fused pipeline times the
 // cell-binned deposit, the direct-atomic deposit and the binned deposit after a presort by
-// supertile once on a prefix of the particles (all overwrite the grid, so the trial runs are
+// supertile once, after one untimed run of each (all overwrite the grid, so the trial runs are
 // harmless), keeps the fastest and caches the choice.  pyl_tune_export / pyl_tune_import move the cache as text ("dims scheme prec
 // hasw nlog choice" lines; choice 0 = binned, 1 = atomic, 2 = presorted binned).
 namespace {
@@ -302,9 +302,9 @@ static int tune_choice(const float* pos, const float* W, int64_t n, int dims, do
     for (auto& x : e) PYL_HIP(hipEventCreate(&x));
     float ms[3] = {0.f, 0.f, 0.f};
     int rc = OK;
-    // the trials deposit a prefix of the particles (a quarter of them, at least 2^22): the same
-    // density of particles per tile for unordered inputs, a slab of the box for ordered ones
-    const int64_t ns = n > (1LL << 22) ? std::max<int64_t>(1LL << 22, n / 4) : n;
+    // (a prefix of the particles would not do: in spatially ordered inputs it is a slab of the box,
+    // whose tiles hold several times the particles the bucket capacities are sized for)
+    const int64_t ns = n;
     for (int c = 0; c < 3 && rc == OK; ++c) {
         rc = deposit_delta(pos, W, ns, dims, BoxSize, scheme, f64, c, grid, dmass, st, rsd, flags);  // warm
         if (rc) break;
