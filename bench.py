@@ -252,6 +252,9 @@ def main():
     ap.add_argument("--realizations", type=int, default=16)
     ap.add_argument("--no-extra", action="store_true",
                     help="default workload only: skip the batch, adjoint and bispectrum figures")
+    ap.add_argument("--only-fp32", action="store_true",
+                    help="profiling runs: time the float32 step only (no float64 path, host drop-in, other "
+                         "workloads or CPU leg), so that the last --steps launches of each kernel are the timed steps")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -318,6 +321,12 @@ def main():
     from pylians_amd.Pk_library import check_number_modes
     check_number_modes(np.concatenate([[1.0], gpu_pk.Nmodes3D]), N)
     print("# fp32c: %.3f ms/step (wall %.3f ms/step)" % (ms, wall * 1e3 / args.steps), file=sys.stderr, flush=True)
+
+    if args.only_fp32:
+        print(json.dumps({"metric": METRIC, "value": round(n_p / (ms * 1e-3) / 1e6, 1), "unit": "Mparticles/s",
+                          "ms_per_step": round(ms, 4), "steps": args.steps,
+                          "stage_ms": {k: round(v[1] / v[0], 4) for k, v in stages.items() if v[0] > 0}}), flush=True)
+        return
 
     # ---- plain float64 GPU path (same particles, float64 grid, FFT and sums)
     step(_lib.F64)
