@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 
 #include "fft.hpp"
 #include "fft_engine.hpp"
@@ -252,39 +253,49 @@ constexpr int hist_bytes(int N, int NL, bool fixk) {
     return (isqrt_c(3LL * (N / 2) * (N / 2)) + 1) * (4 * 8 + 4) + (N / 2 + 1) * 12 +
            (fixk ? NL : 1) * (isqrt_c(2LL * (N / 2) * (N / 2)) + 1) * 12;
 }
-// lines per block for the binning kernel: at most 8, and few enough that two blocks share a CU
-// (160 KB of LDS) beside their histograms
-// (lines with a radix-3 factor from 192 points up: twice the threads, measured faster)
-constexpr int threads_bin_c(int N, int NL) {
-    return clamp_i(N * NL / 16 * (N % 3 == 0 && N >= 192 ? 2 : 1), 64, 512);
+// Threads of the binning kernel: N/16 per column line, doubled (measured faster) for lines with a
+// radix-3 factor from 192 points up and where bin_wide says so: blocks that own a kz block (line of
+// sight z) hold a 2-D histogram row per line, which leaves room for few blocks per CU, and from 256
+// points up twice the waves per block paid (0.73 -> 0.55 ms at 512); with the line of sight along x
+// or y, 256 points gained (0.094 -> 0.085 ms) and 512 lost (0.397 -> 0.410 ms).
+constexpr bool bin_wide(int N, bool fixk) { return fixk ? N >= 256 : (N >= 64 && N <= 256); }
+constexpr int threads_bin_c(int N, int NL, bool fixk) {
+    return clamp_i(N * NL / 16 * ((N % 3 == 0 && N >= 192) || bin_wide(N, fixk) ? 2 : 1), 64, 512);
 }
+// LDS rows of the binning kernel's transform (RowMap::ROWS for its EU), at least N + T / NL
+constexpr int bin_rows(int N, int T, int NL) {
+    const int eu = (N / 2 + 1) / (T / NL) > 0 ? (N / 2 + 1) / (T / NL) : 1;
+    const int rows = eu % 2 == 0 ? N + 2 * ((N / 2) / eu) + 2 : N;
+    return rows > N + T / NL ? rows : N + T / NL;
+}
+// lines per block: at most 8, and few enough that two blocks share a CU (160 KB of LDS) beside their
+// histograms
 template <typename R, int N, bool FIXK> constexpr int nl_bin() {
     int nl = 8;
-    while (nl > 2 && (N + threads_bin_c(N, nl) / nl) * nl * (int)sizeof(cplx<R>) +
+    while (nl > 2 && bin_rows(N, threads_bin_c(N, nl, FIXK), nl) * nl * (int)sizeof(cplx<R>) +
                              N * (int)(sizeof(cplx<R>) + sizeof(R)) + hist_bytes(N, nl, FIXK) > 80 * 1024)
         nl /= 2;
     return nl;
 }
-template <int N, int NL> constexpr int threads_bin() { return threads_bin_c(N, NL); }
 
 
 // Squared phase atan2(re, |v|)^2 of a mode (Pk_library.pyx:374), r2 = |v|^2 >= re^2.  float32:
-// t = |re|/|v| in [0, 1], atan(t) = pi/4 + atan((t-1)/(t+1)) above tan(pi/8), then atan(x) =
-// x P(x^2) with a degree-4 polynomial fitted (Lawson-weighted least squares) to atan(x)/x on
-// [0, tan(pi/8)]: 1.9e-7 relative error in float32.  float64: the library atan2.
+// t = |re|/|v| in [0, 1] and atan(t) = t P(t^2) with a degree-7 polynomial fitted (Lawson-weighted
+// least squares) to atan(t)/t on the whole of [0, 1], so there is no range reduction and no branch:
+// 2.1e-7 relative error in float32.  r2 = 0 (then re = 0) gives 0.  float64: the library atan2.
 __device__ __forceinline__ float phase_sq(float re, float r2) {
-    if (r2 == 0.0f) return 0.0f;
-    const float t = fminf(fabsf(re) * __builtin_amdgcn_rsqf(r2), 1.0f);
-    const bool big = t > 0.41421356f;
-    const float x = big ? (t - 1.0f) * __builtin_amdgcn_rcpf(t + 1.0f) : t;
-    const float s = x * x;
-    float p = 0.07986736f;
-    p = p * s - 0.13852088f;
-    p = p * s + 0.19974470f;
-    p = p * s - 0.33332798f;
-    p = p * s + 1.0f;
-    float a = x * p;
-    if (big) a += 0.78539816f;
+    // t may exceed 1 by the rounding of rsq; the polynomial holds there too
+    const float t = fabsf(re) * __builtin_amdgcn_rsqf(r2 + 1.17549435e-38f);
+    const float s = t * t;
+    float p = -0.004693399649f;
+    p = p * s + 0.024252869189f;
+    p = p * s - 0.059487096965f;
+    p = p * s + 0.099143467844f;
+    p = p * s - 0.140195026994f;
+    p = p * s + 0.199697285891f;
+    p = p * s - 0.333319902420f;
+    p = p * s + 0.999999880791f;
+    const float a = t * p;
     return a * a;
 }
 __device__ __forceinline__ double phase_sq(double re, double r2) {
@@ -293,7 +304,8 @@ __device__ __forceinline__ double phase_sq(double re, double r2) {
 }
 
 // LDS row of FFT output k in the binning kernel.  The epilogue pairs rows k and N-k (same |k|);
-// thread jrow of a column owns the pairs u in [jrow*EU, (jrow+1)*EU).  With EU odd, the rows of the
+// thread jrow of a column owns the EU pairs from u = jrow*EU (the last few threads one more, each
+// shifted by one per thread before it that took one more).  With EU odd, the rows of the
 // 8 threads of a wave that share a step are an odd number of 64-byte rows apart and spread over all
 // LDS banks; with EU even, one spare row per group restores that.
 template <int N, int EU> struct RowMap {
@@ -369,10 +381,11 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
     const int RS = A.RS;
     constexpr int TPC = T / NL;              // threads per column
     constexpr int U = N / 2 + 1;             // row pairs (k, N-k), k = 0 .. N/2 (k = 0, N/2 alone)
-    constexpr int EU = (U + TPC - 1) / TPC;  // consecutive pairs per thread in the epilogue
+    constexpr int EU = U / TPC;              // consecutive pairs per thread; the last UR threads of a
+    constexpr int UR = U % TPC;              // column take one more
     static_assert(T % NL == 0, "bad binning geometry");
     using F = First<N, NL, T>;
-    using RM = RowMap<N, EU>;
+    using RM = RowMap<N, (EU > 0 ? EU : 1)>;
     constexpr bool ONE = F::NST == 1;
     __shared__ cplx<R> s[(RM::ROWS > N ? RM::ROWS : N) * NL];
     __shared__ cplx<R> tw[N];
@@ -465,6 +478,10 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
         }
         int u, kzz;
         item(it, u, kzz);
+        // the first kz of the column block (the same for the whole block): only the blocks that hold
+        // kz = 0 or kz = N/2 apply the independent-mode rule of those planes (Pk_library.pyx:322-330)
+        const int kzlo = kzz - l;
+        const bool zblock = kzlo == 0 || (even && middle >= kzlo && middle < kzlo + NL);
         if (kzz >= W || (drop0 && kzz == 0)) continue;
         const int kz = kzz;
         const int ku = kfreq(u, N);
@@ -476,86 +493,102 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
         const int cper = c0 - kps * kps;
         const R kps2 = (R)(kps * kps);
         const long long kbase = (long long)u * KUST + kzz;
-        int cur3 = -1, cur2 = -1;
+        // |k|^2 = c0 + u^2 and k_perp^2 = cper + u^2 of this thread's first pair and their bins
+        // floor(sqrt); each step of u adds at most 1 to sqrt(c + u^2) (its derivative u / sqrt(c + u^2)
+        // is <= 1), so each bin grows by 0 or 1 per step and is carried along without a square root
+        const int extra = jrow - (TPC - UR);   // >= 0: this thread takes EU + 1 pairs
+        const int uq0 = jrow * EU + (extra > 0 ? extra : 0);
+        const int ne = EU + (extra >= 0 ? 1 : 0);
+        int k2 = c0 + uq0 * uq0, kp2 = cper + uq0 * uq0;
+        int bin = isqrt24(k2), wb = isqrt24(kp2);
+        const int w2o = fixk ? l * nper : 0;
+        // the current runs: 3-D bin cur3 and 2-D entry cur2 with their sums and modes; a run is added
+        // to the LDS histograms when the bin changes and at the end (if it holds modes)
+        int cur3 = bin, cur2 = w2o + wb;
         double a0 = 0.0, a2 = 0.0, a4 = 0.0, ap = 0.0, b1 = 0.0, b2 = 0.0;
-        unsigned m3 = 0u, m1 = 0u, m2 = 0u;   // modes of the current runs
-        // pairs of rows with opposite k_j: the same |k|, k_perp, mu^2 and window factor
-        for (int q = 0; q < EU; ++q) {
-            const int uu = jrow * EU + q;
-            if (U % TPC != 0 && uu >= U) break;
-            const int ja = uu, jb = (N - uu) % N;
-            const bool pair = jb != ja;
-            const cplx<R> va = s[RM::row(ja) * NL + l];
-            const cplx<R> vb = pair ? s[RM::row(jb) * NL + l] : mk<R>(0, 0);
-            // independent modes on the kz = 0 and Nyquist planes (Pk_library.pyx:322-330)
-            bool ia = true, ib = pair;
-            if (zplane) {
-                const int xa = last == 0 ? uu : ku, ya = last == 0 ? ku : uu;
-                const int xb = last == 0 ? -uu : ku, yb = last == 0 ? ku : -uu;
-                // k_j = N/2 is +N/2 (kfreq); -uu for the partner row N - uu
-                ia = !(xa < 0 || ((xa == 0 || (xa == middle && even)) && ya < 0));
-                ib = pair && !(xb < 0 || ((xb == 0 || (xb == middle && even)) && yb < 0));
-            }
-            const R fac = win[uu] * cuz;
-            if (keep) {
-                keep[kbase + ja * KEST] = ia ? scale(va, fac) : va;
-                if (pair) keep[kbase + jb * KEST] = ib ? scale(vb, fac) : vb;
-            }
-            if (!ia && !ib) continue;
-            const unsigned nm = (ia ? 1u : 0u) + (ib ? 1u : 0u);
-            const R ra = va.x * va.x + va.y * va.y;
-            const R rb = vb.x * vb.x + vb.y * vb.y;
-            const R r2 = (ia ? ra : (R)0) + (ib ? rb : (R)0);
-            const int k2 = c0 + uu * uu;
-            const int bin = isqrt24(k2);
-            const R mu2 = k2 == 0 ? (R)0 : kps2 * recip((R)k2);
-            const R d2 = r2 * (fac * fac);
-            const R L2 = ((R)1.5 * mu2 - (R)0.5);
-            const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
-            if (bin != cur3) {
-                if (cur3 >= 0) {
+        unsigned m3 = 0u, m1 = 0u, m2 = 0u;
+        // pairs of rows with opposite k_j: the same |k|, k_perp, mu^2 and window factor.  ZP: the
+        // column block holds a kz = 0 or Nyquist column, where a mode may not be independent
+        auto pairs = [&](auto zp) {
+            constexpr bool ZP = decltype(zp)::value;
+#pragma unroll 1
+            for (int q = 0; q < ne; ++q) {
+                const int uu = uq0 + q;
+                if (q > 0) {
+                    k2 += 2 * uu - 1;
+                    kp2 += 2 * uu - 1;
+                    bin += __mul24(bin + 1, bin + 1) <= k2 ? 1 : 0;
+                    wb += __mul24(wb + 1, wb + 1) <= kp2 ? 1 : 0;
+                }
+                const int ja = uu, jb = (N - uu) % N;
+                const bool pair = jb != ja;
+                const cplx<R> va = s[RM::row(ja) * NL + l];
+                const cplx<R> vb = pair ? s[RM::row(jb) * NL + l] : mk<R>(0, 0);
+                bool ia = true, ib = pair;
+                if (ZP && zplane) {
+                    const int xa = last == 0 ? uu : ku, ya = last == 0 ? ku : uu;
+                    const int xb = last == 0 ? -uu : ku, yb = last == 0 ? ku : -uu;
+                    // k_j = N/2 is +N/2 (kfreq); -uu for the partner row N - uu
+                    ia = !(xa < 0 || ((xa == 0 || (xa == middle && even)) && ya < 0));
+                    ib = pair && !(xb < 0 || ((xb == 0 || (xb == middle && even)) && yb < 0));
+                }
+                const R fac = win[uu] * cuz;
+                if (keep) {
+                    keep[kbase + ja * KEST] = ia ? scale(va, fac) : va;
+                    if (pair) keep[kbase + jb * KEST] = ib ? scale(vb, fac) : vb;
+                }
+                if (ZP && !ia && !ib) continue;
+                const unsigned nm = (ia ? 1u : 0u) + (ib ? 1u : 0u);
+                const R ra = va.x * va.x + va.y * va.y;
+                const R rb = vb.x * vb.x + vb.y * vb.y;
+                const R r2 = (ia ? ra : (R)0) + (ib ? rb : (R)0);
+                const R mu2 = k2 == 0 ? (R)0 : kps2 * recip((R)k2);
+                const R d2 = r2 * (fac * fac);
+                const R L2 = ((R)1.5 * mu2 - (R)0.5);
+                const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
+                if (bin != cur3) {
                     atomicAdd(&h3[4 * cur3 + 0], a0);
                     atomicAdd(&h3[4 * cur3 + 1], a2);
                     atomicAdd(&h3[4 * cur3 + 2], a4);
                     if (doph) atomicAdd(&h3[4 * cur3 + 3], ap);
                     atomicAdd(&c3[cur3], m3);
+                    cur3 = bin;
+                    a0 = a2 = a4 = ap = 0.0;
+                    m3 = 0u;
                 }
-                cur3 = bin;
-                a0 = a2 = a4 = ap = 0.0;
-                m3 = 0u;
-            }
-            const double dd = (double)d2;
-            m3 += nm;
-            a0 += dd;
-            a2 += (double)(d2 * L2);
-            a4 += (double)(d2 * L4);
-            if (doph) {
-                // the phase of a mode is unchanged by the (positive) window factor
-                R pp = (R)0;
-                if (ia) pp += phase_sq(va.x, ra);
-                if (ib) pp += phase_sq(vb.x, rb);
-                ap += (double)pp;
-            }
-            if (do1 && k2 <= middle * middle) {
-                b1 += dd;
-                m1 += nm;
-            }
-            if (do2) {
-                const int w2 = (fixk ? l * nper : 0) + isqrt24(cper + uu * uu);
-                if (w2 != cur2) {
-                    if (cur2 >= 0) {
+                const double dd = (double)d2;
+                m3 += nm;
+                a0 += dd;
+                a2 += (double)(d2 * L2);
+                a4 += (double)(d2 * L4);
+                if (doph) {
+                    // the phase of a mode is unchanged by the (positive) window factor
+                    const R pa = phase_sq(va.x, ra), pb = phase_sq(vb.x, rb);
+                    ap += (double)((ia ? pa : (R)0) + (ib ? pb : (R)0));
+                }
+                if (do1 && k2 <= middle * middle) {
+                    b1 += dd;
+                    m1 += nm;
+                }
+                if (do2) {
+                    const int w2 = w2o + wb;
+                    if (w2 != cur2) {
                         atomicAdd(&h2[cur2], b2);
                         atomicAdd(&c2[cur2], m2);
+                        cur2 = w2;
+                        b2 = 0.0;
+                        m2 = 0u;
                     }
-                    cur2 = w2;
-                    b2 = 0.0;
-                    m2 = 0u;
+                    b2 += dd;
+                    m2 += nm;
                 }
-                b2 += dd;
-                m2 += nm;
             }
-        }
-        if (cur3 >= 0) {
+        };
+        if (zblock)
+            pairs(std::true_type{});
+        else
+            pairs(std::false_type{});
+        if (m3 != 0u) {
             atomicAdd(&h3[4 * cur3 + 0], a0);
             atomicAdd(&h3[4 * cur3 + 1], a2);
             atomicAdd(&h3[4 * cur3 + 2], a4);
@@ -566,7 +599,7 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
             atomicAdd(&h1[abs(kps)], b1);
             atomicAdd(&c1[abs(kps)], m1);
         }
-        if (cur2 >= 0) {
+        if (m2 != 0u) {
             atomicAdd(&h2[cur2], b2);
             atomicAdd(&c2[cur2], m2);
         }
@@ -694,7 +727,7 @@ int pk_last_axis(int axis) { return axis == 0 ? 1 : 0; }
 // column blocks with prefetch (four per CU with fixk measured faster)
 template <typename R, int N, bool FIXK> struct BinShape {
     static constexpr int NL = nl_bin<R, N, FIXK>();
-    static constexpr int T = threads_bin<N, NL>();
+    static constexpr int T = threads_bin_c(N, NL, FIXK);
     static constexpr int W = N / 2 + 1;   // kz = 0 .. N/2; rows of the spectrum may be padded to P >= W
     int nchunk, grid, n2w, RS, n31;
     long long n2;
