@@ -68,6 +68,21 @@ extern "C" {
 int pyl_abi_version(void);
 const char* pyl_last_error(void);
 
+/* ---------------------------------------------------------------- device memory
+ * No counterpart in the reference (its arrays live in host memory): the few device-memory calls a
+ * caller without a device-array library needs to use the entry points below from host arrays (the
+ * numpy drop-in path).  pyl_device_count reports 0 without a GPU or driver; pyl_memcpy_h2d,
+ * pyl_memcpy_d2d and pyl_dev_zero are asynchronous on `stream` (NULL: the default stream);
+ * pyl_memcpy_d2h returns when the copy is done. */
+int pyl_device_count(int* count);
+int pyl_dev_alloc(void** ptr, int64_t bytes);
+int pyl_dev_free(void* ptr);
+int pyl_memcpy_h2d(void* dst, const void* src, int64_t bytes, void* stream);
+int pyl_memcpy_d2h(void* dst, const void* src, int64_t bytes, void* stream);
+int pyl_memcpy_d2d(void* dst, const void* src, int64_t bytes, void* stream);
+int pyl_dev_zero(void* ptr, int64_t bytes, void* stream);
+int pyl_stream_sync(void* stream);
+
 /* ---------------------------------------------------------------- mass assignment
  * Replaces MAS_library.MA and the per-scheme Cython functions NGP/CIC/TSC/PCS, NGPW/CICW/TSCW/PCSW,
  * CICW_d/NGPW_d (MAS_library.pyx:57-545).  pos: (n, axes) float32 row-major; W: (n,) float32 or

@@ -15,7 +15,6 @@ expected mass (or count), and the time taken.
 import time
 
 import numpy as np
-import torch
 
 from . import _lib, staging
 from .MAS_library import MA
@@ -29,7 +28,7 @@ def density_field_gadget(snapshot_fname, ptypes, dims, MAS="CIC", do_RSD=False, 
     if verbose:
         print("\nComputing density field of particles", ptypes)
     snap = staging.Staged(snapshot_fname, staging.expand_types(ptypes))
-    grid = torch.zeros((dims, dims, dims), dtype=torch.float32, device="cuda")
+    grid = _lib.zeros((dims, dims, dims), _lib.F32T)
     if snap.rows:
         box = snap.box
         pos = snap.positions()
@@ -38,7 +37,7 @@ def density_field_gadget(snapshot_fname, ptypes, dims, MAS="CIC", do_RSD=False, 
         counted = len(snap.types) == 1 and snap.mass_table[snap.types[0]] != 0
         W = None if counted else snap.weights()
         MA(pos, grid, box, MAS, W=W)
-        expected = float(snap.rows) if counted else float(W.double().sum())
+        expected = float(snap.rows) if counted else float(np.sum(_lib.to_host(W), dtype=np.float64))
         del pos, W
     else:
         expected = 0.0

@@ -18,7 +18,6 @@ unknown scheme prints and raises SystemExit (MAS_library.pyx:64-66, 81-82).
 import sys
 
 import numpy as np
-import torch
 
 from . import _lib
 
@@ -35,22 +34,21 @@ def FLOAT_type():
 
 def _deposit(pos, number, BoxSize, scheme, W, flags, grid_f64=False, grid_view=None):
     """Run pyl_mas_deposit accumulating into `number` in place (numpy or device tensor)."""
-    gdtype = torch.float64 if grid_f64 else torch.float32
     want = np.float64 if grid_f64 else np.float32
     axes = int(pos.shape[1])
     if _lib.is_dev(number):
-        if number.dtype != gdtype or not number.is_contiguous():
-            raise ValueError("number must be a contiguous %s device tensor" % gdtype)
+        if _lib.dtype_of(number) != want or not number.is_contiguous():
+            raise ValueError("number must be a contiguous %s device tensor" % np.dtype(want).name)
         g = number
     else:
         if not isinstance(number, np.ndarray) or number.dtype != want:
             raise ValueError("Buffer dtype mismatch, expected '%s' but got '%s'"
                              % (np.dtype(want).name, getattr(number, "dtype", type(number))))
-        g = _lib.to_dev(number, gdtype)
-    p = _lib.to_dev(pos, torch.float32)
-    w = None if W is None else _lib.to_dev(W, torch.float32)
+        g = _lib.to_dev(number, want)
+    p = _lib.to_dev(pos, _lib.F32T)
+    w = None if W is None else _lib.to_dev(W, _lib.F32T)
     n = int(p.shape[0])
-    if w is not None and int(w.numel()) != n:
+    if w is not None and int(np.prod(w.shape)) != n:
         raise ValueError("W must have one weight per particle")
     dims = int(g.shape[0])
     f = flags | (_lib.GRID_F64 if grid_f64 else 0)
@@ -158,10 +156,10 @@ PCSc2D, PCSWc2D = _c_path("PCS")
 
 def CIC_interp(density, BoxSize, pos, den):
     """CIC interpolation of a 3-D float32 grid at particle positions into `den` (MAS_library.pyx:558-599)."""
-    g = _lib.to_dev(density, torch.float32)
-    p = _lib.to_dev(pos, torch.float32)
+    g = _lib.to_dev(density, _lib.F32T)
+    p = _lib.to_dev(pos, _lib.F32T)
     n = int(p.shape[0])
-    out = den if (_lib.is_dev(den) and den.dtype == torch.float32 and den.is_contiguous()) else _lib.empty((n,), torch.float32)
+    out = den if (_lib.is_dev(den) and _lib.dtype_of(den) == _lib.F32T and den.is_contiguous()) else _lib.empty((n,), _lib.F32T)
     _lib.check(_lib.lib().pyl_interp(_lib.ptr(g), int(g.shape[0]), float(BoxSize), _lib.ptr(p), n, 2,
                                      _lib.ptr(out), _lib.stream()))
     if out is not den:

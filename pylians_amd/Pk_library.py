@@ -21,7 +21,6 @@ import sys
 import time
 
 import numpy as np
-import torch
 
 from . import _lib
 
@@ -126,22 +125,22 @@ def _fft_c2r(a, real_t, cplx_t, prec):
 
 def FFT3Dr_f(a, threads=1):
     """Unnormalised r2c FFT of a float32 (N,N,N) field -> complex64 (N,N,N//2+1) (Pk_library.pyx:120-133)."""
-    return _fft_r2c(a, torch.float32, torch.complex64, _lib.F32)
+    return _fft_r2c(a, _lib.F32T, _lib.C64T, _lib.F32)
 
 
 def FFT3Dr_d(a, threads=1):
     """float64 version (Pk_library.pyx:136-149)."""
-    return _fft_r2c(a, torch.float64, torch.complex128, _lib.F64)
+    return _fft_r2c(a, _lib.F64T, _lib.C128T, _lib.F64)
 
 
 def IFFT3Dr_f(a, threads=1):
     """1/N^3-normalised c2r FFT complex64 (N,N,N//2+1) -> float32 (N,N,N) (Pk_library.pyx:152-165)."""
-    return _fft_c2r(a, torch.float32, torch.complex64, _lib.F32)
+    return _fft_c2r(a, _lib.F32T, _lib.C64T, _lib.F32)
 
 
 def IFFT3Dr_d(a, threads=1):
     """float64 version (Pk_library.pyx:168-181)."""
-    return _fft_c2r(a, torch.float64, torch.complex128, _lib.F64)
+    return _fft_c2r(a, _lib.F64T, _lib.C128T, _lib.F64)
 
 
 def pk_layout(dims):
@@ -184,12 +183,12 @@ class Pk:
         start = time.time()
         if VERBOSE:
             print("\nComputing power spectrum of the field...")
-        f64 = (delta.dtype == torch.float64) if isinstance(delta, torch.Tensor) else (delta.dtype == np.float64)
-        real_t, cplx_t = (torch.float64, torch.complex128) if f64 else (torch.float32, torch.complex64)
+        f64 = _lib.dtype_of(delta) == np.float64
+        real_t, cplx_t = (_lib.F64T, _lib.C128T) if f64 else (_lib.F32T, _lib.C64T)
         d = _grid3(delta, real_t, "delta")
         dims = int(d.shape[0])
         _, _, _, nout = pk_layout(dims)
-        out = _lib.empty((nout,), torch.float64)
+        out = _lib.empty((nout,), _lib.F64T)
         dk = _lib.empty((dims, dims, dims // 2 + 1), cplx_t) if keep_deltak else None
         rc = _lib.lib().pyl_pk(_lib.ptr(d), dims, float(BoxSize), int(axis), MAS_function(MAS),
                                _lib.F64 if f64 else _lib.F32, _lib.PK_ALL | _EXTRA_FLAGS, _lib.ptr(out),
@@ -220,11 +219,10 @@ class XPk:
             if int(d.shape[0]) != dims:
                 print("Fields have different grid sizes!!!")
                 sys.exit()
-        f64 = all(((x.dtype == torch.float64) if isinstance(x, torch.Tensor) else (np.asarray(x).dtype == np.float64))
-                  for x in delta)
-        rt = torch.float64 if f64 else torch.float32
+        f64 = all(_lib.dtype_of(x if hasattr(x, "dtype") else np.asarray(x)) == np.float64 for x in delta)
+        rt = _lib.F64T if f64 else _lib.F32T
         devs = [_grid3(x, rt, "delta") for x in delta]
-        ptrs = (ctypes.c_void_p * fields)(*[t.data_ptr() for t in devs])
+        ptrs = (ctypes.c_void_p * fields)(*[_lib.ptr(t).value for t in devs])
         mas = (ctypes.c_int * fields)(*[MAS_function(m) for m in MAS])
         n = ctypes.c_int64()
         _lib.check(_lib.lib().pyl_xpk_layout(dims, fields, ctypes.byref(n)))
@@ -267,8 +265,8 @@ class Bk:
 
     def __init__(self, delta, BoxSize, k1, k2, theta, MAS="CIC", threads=1):
         import ctypes
-        f64 = (delta.dtype == torch.float64) if isinstance(delta, torch.Tensor) else (delta.dtype == np.float64)
-        d = _grid3(delta, torch.float64 if f64 else torch.float32, "delta")
+        f64 = _lib.dtype_of(delta) == np.float64
+        d = _grid3(delta, _lib.F64T if f64 else _lib.F32T, "delta")
         dims = int(d.shape[0])
         theta = np.ascontiguousarray(theta, dtype=np.float64)
         bins = theta.shape[0]
@@ -290,8 +288,8 @@ def correct_MAS(delta, BoxSize, MAS="CIC", threads=1):
     """The field with its mass-assignment window removed (Pk_library.pyx:1749-1806): the independent
     modes of FFT(delta) times prod_i (x_i/sin x_i)^p, back to real space (1/N^3-normalised), on the
     GPU.  float32 like the reference; float64 input stays float64."""
-    f64 = (delta.dtype == torch.float64) if isinstance(delta, torch.Tensor) else (delta.dtype == np.float64)
-    real_t = torch.float64 if f64 else torch.float32
+    f64 = _lib.dtype_of(delta) == np.float64
+    real_t = _lib.F64T if f64 else _lib.F32T
     d = _grid3(delta, real_t, "delta")
     dims = int(d.shape[0])
     out = _lib.empty((dims, dims, dims), real_t)
@@ -309,8 +307,8 @@ class Xi:
     def __init__(self, delta, BoxSize, MAS="CIC", axis=2, threads=1):
         import ctypes
         BoxSize = float(np.float32(BoxSize))   # a C float in the reference (:2036)
-        f64 = (delta.dtype == torch.float64) if isinstance(delta, torch.Tensor) else (delta.dtype == np.float64)
-        d = _grid3(delta, torch.float64 if f64 else torch.float32, "delta")
+        f64 = _lib.dtype_of(delta) == np.float64
+        d = _grid3(delta, _lib.F64T if f64 else _lib.F32T, "delta")
         dims = int(d.shape[0])
         kmax = ctypes.c_int()
         n = ctypes.c_int64()

@@ -22,7 +22,6 @@ CIC deconvolution).
 import os
 
 import numpy as np
-import torch
 
 from . import _lib, staging
 from .MAS_library import MA
@@ -60,8 +59,12 @@ def _write_table(fname, k, P, Nmodes):
 
 
 def _print_ranges(pos):
-    lo = pos.min(dim=0).values.cpu().numpy()
-    hi = pos.max(dim=0).values.cpu().numpy()
+    if _lib.is_tensor(pos):
+        lo = pos.min(dim=0).values.cpu().numpy()
+        hi = pos.max(dim=0).values.cpu().numpy()
+    else:
+        host = _lib.to_host(pos)
+        lo, hi = host.min(axis=0), host.max(axis=0)
     for c, name in enumerate("XYZ"):
         print("%.3f < %s [Mpc/h] < %.3f%s" % (lo[c], name, hi[c], "\n" if c == 2 else ""))
 
@@ -93,7 +96,7 @@ def _overdensity(snapshot_fname, ptype, dims, do_RSD, axis):
     pos = snap.positions()
     if do_RSD:
         pos_redshift_space(pos, snap.velocities(), snap.box, snap.hubble_z, snap.head.redshift, axis)
-    grid = torch.zeros((dims, dims, dims), dtype=torch.float32, device="cuda")
+    grid = _lib.zeros((dims, dims, dims), _lib.F32T)
     MA(pos, grid, snap.box, "CIC")
     del pos
     _lib.check(_lib.lib().pyl_field_div_sub(_lib.ptr(grid), int(grid.numel()), snap.rows / dims ** 3, 1.0,

@@ -13,7 +13,6 @@ device output vector; with ``flags | PK_ASYNC`` no host synchronisation (the cal
 the returned Nmodes, which the binning kernel counted).
 """
 import numpy as np
-import torch
 
 from . import _lib
 from .Pk_library import pk_layout, unpack_pk
@@ -45,20 +44,20 @@ class PkParticles:
         (redshift_space_library.pyx:29-43), fused into the deposit; `pos` is not modified."""
         if MAS not in _lib.SCHEMES:
             raise ValueError("unknown MAS %r" % (MAS,))
-        p = _lib.to_dev(pos, torch.float32)
+        p = _lib.to_dev(pos, _lib.F32T)
         if p.ndim != 2 or p.shape[1] != 3:
             raise ValueError("pos must be (n, 3)")
-        w = None if W is None else _lib.to_dev(W, torch.float32)
+        w = None if W is None else _lib.to_dev(W, _lib.F32T)
         rsd = None
         if vel is not None:
             if Hubble is None:
                 raise ValueError("redshift space needs Hubble")
-            v = _lib.to_dev(vel, torch.float32)
+            v = _lib.to_dev(vel, _lib.F32T)
             if tuple(v.shape) != tuple(p.shape):
                 raise ValueError("vel must have the shape of pos")
             rsd = (v, Hubble, redshift, axis if rsd_axis is None else rsd_axis)
         _, _, _, nout = pk_layout(dims)
-        out = _lib.empty((nout,), torch.float64)
+        out = _lib.empty((nout,), _lib.F64T)
         f = flags | {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED, "presorted": _lib.SORTED}[deposit]
         try:
             run_device(p, w, out, BoxSize, dims, _lib.SCHEMES[MAS], axis, PRECISION[precision], f, rsd)

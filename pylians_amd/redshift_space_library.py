@@ -7,7 +7,6 @@ C-contiguous float32 (n, cols) arrays, as the reference's typed memoryviews requ
 are shifted on the GPU and written back, device tensors are shifted where they lie.
 """
 import numpy as np
-import torch
 
 from . import _lib
 
@@ -16,7 +15,7 @@ __all__ = ["pos_redshift_space"]
 
 def _check(a, name):
     if _lib.is_dev(a):
-        if a.dtype != torch.float32 or not a.is_contiguous() or a.ndim != 2:
+        if _lib.dtype_of(a) != np.float32 or not a.is_contiguous() or a.ndim != 2:
             raise ValueError("%s must be a contiguous (n, cols) float32 device tensor" % name)
     elif not isinstance(a, np.ndarray) or a.dtype != np.float32:
         raise ValueError("Buffer dtype mismatch, expected 'float' but got '%s'" % getattr(a, "dtype", type(a)))
@@ -33,8 +32,8 @@ def pos_redshift_space(pos, vel, BoxSize, Hubble, redshift, axis):
     n, cols = int(pos.shape[0]), int(pos.shape[1])
     if not 0 <= axis < cols:
         raise IndexError("axis %d out of range for %d columns" % (axis, cols))
-    p = pos if _lib.is_dev(pos) else _lib.to_dev(pos, torch.float32)
-    v = _lib.to_dev(vel, torch.float32)
+    p = pos if _lib.is_dev(pos) else _lib.to_dev(pos, _lib.F32T)
+    v = _lib.to_dev(vel, _lib.F32T)
     _lib.check(_lib.lib().pyl_pos_redshift_space(_lib.ptr(p), _lib.ptr(v), n, cols, float(BoxSize), float(Hubble),
                                                  float(redshift), int(axis), _lib.stream()))
     if not _lib.is_dev(pos):

@@ -1,7 +1,6 @@
 """GPU drop-in of smoothing_library (smoothing_library.pyx:20-114): filters on the grid and
 field smoothing through the same r2c / c2r passes as Pk_library."""
 import numpy as np
-import torch
 
 from . import _lib
 
@@ -17,7 +16,7 @@ def FT_filter(BoxSize, R, dims, Filter, threads=1):
     if Filter not in _FILTERS:
         raise Exception("Filter %s not implemented!" % Filter)
     dims = int(dims)
-    out = _lib.empty((dims, dims, dims // 2 + 1), torch.complex64)
+    out = _lib.empty((dims, dims, dims // 2 + 1), _lib.C64T)
     _lib.check(_lib.lib().pyl_ft_filter(dims, float(BoxSize), float(R), _FILTERS[Filter], _lib.ptr(out),
                                         _lib.stream()))
     return _lib.to_host(out)
@@ -28,11 +27,11 @@ def field_smoothing(field, filter_k, threads=1):
     float32; numpy in, numpy out (device tensors stay on the device)."""
     if field.shape[0] != filter_k.shape[0]:
         raise Exception("field and filter have different grids!!!")
-    f = _lib.to_dev(field, torch.float32)
-    k = _lib.to_dev(filter_k, torch.complex64)
+    f = _lib.to_dev(field, _lib.F32T)
+    k = _lib.to_dev(filter_k, _lib.C64T)
     dims = int(f.shape[0])
     if f.ndim != 3 or tuple(k.shape) != (dims, dims, dims // 2 + 1):
         raise ValueError("field must be (dims, dims, dims) and filter_k (dims, dims, dims//2+1)")
-    out = _lib.empty((dims, dims, dims), torch.float32)
+    out = _lib.empty((dims, dims, dims), _lib.F32T)
     _lib.check(_lib.lib().pyl_field_smoothing(_lib.ptr(f), _lib.ptr(k), _lib.ptr(out), dims, _lib.stream()))
     return out if _lib.is_dev(field) else _lib.to_host(out)

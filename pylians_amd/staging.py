@@ -14,7 +14,6 @@ one per file and type.  Conventions of the reference's readers that the results 
 import math
 
 import numpy as np
-import torch
 
 from . import _lib, readgadget
 
@@ -58,10 +57,9 @@ class Staged:
     def _fill(self, width, block, convert):
         """One device buffer of `rows` x `width` float32 filled segment by segment from the files."""
         shape = (self.rows, width) if width > 1 else (self.rows,)
-        dev = _lib.empty(shape, torch.float32)
+        dev = _lib.empty(shape, _lib.F32T)
         for fname, t, first, n in self.segments:
-            host = convert(t, readgadget.read_segment(fname, block, t))
-            dev[first:first + n].copy_(torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)))
+            _lib.copy_rows(dev, first, convert(t, readgadget.read_segment(fname, block, t)))
         return dev
 
     def positions(self):
@@ -76,13 +74,12 @@ class Staged:
     def weights(self):
         """(rows,) float32 device masses in Msun/h: the mass table value of a type where it is set,
         the MASS block otherwise."""
-        dev = _lib.empty((self.rows,), torch.float32)
+        dev = _lib.empty((self.rows,), _lib.F32T)
         for fname, t, first, n in self.segments:
             if self.mass_table[t] != 0:
-                dev[first:first + n].fill_(float(np.float32(self.mass_table[t])))
+                _lib.copy_rows(dev, first, np.full(n, self.mass_table[t], dtype=np.float32))
             else:
-                m = readgadget.read_segment(fname, "MASS", t) * np.float32(1e10)
-                dev[first:first + n].copy_(torch.from_numpy(np.ascontiguousarray(m, dtype=np.float32)))
+                _lib.copy_rows(dev, first, readgadget.read_segment(fname, "MASS", t) * np.float32(1e10))
         return dev
 
     def type_mass(self, t):
