@@ -263,6 +263,7 @@ constexpr int threads_bin_c(int N, int NL, bool fixk) {
     return clamp_i(N * NL / 16 * ((N % 3 == 0 && N >= 192) || bin_wide(N, fixk) ? 2 : 1), 64, 512);
 }
 // LDS rows of the binning kernel's transform (RowMap::ROWS for its EU), at least N + T / NL
+// (rows of NL + 1 values in the mapped layout)
 constexpr int bin_rows(int N, int T, int NL) {
     const int eu = (N / 2 + 1) / (T / NL) > 0 ? (N / 2 + 1) / (T / NL) : 1;
     const int rows = eu % 2 == 0 ? N + 2 * ((N / 2) / eu) + 2 : N;
@@ -272,7 +273,7 @@ constexpr int bin_rows(int N, int T, int NL) {
 // histograms
 template <typename R, int N, bool FIXK> constexpr int nl_bin() {
     int nl = 8;
-    while (nl > 2 && bin_rows(N, threads_bin_c(N, nl, FIXK), nl) * nl * (int)sizeof(cplx<R>) +
+    while (nl > 2 && bin_rows(N, threads_bin_c(N, nl, FIXK), nl) * (nl + 1) * (int)sizeof(cplx<R>) +
                              N * (int)(sizeof(cplx<R>) + sizeof(R)) + hist_bytes(N, nl, FIXK) > 80 * 1024)
         nl /= 2;
     return nl;
@@ -304,10 +305,10 @@ __device__ __forceinline__ double phase_sq(double re, double r2) {
 }
 
 // LDS row of FFT output k in the binning kernel.  The epilogue pairs rows k and N-k (same |k|);
-// thread jrow of a column owns the EU pairs from u = jrow*EU (the last few threads one more, each
-// shifted by one per thread before it that took one more).  With EU odd, the rows of the
-// 8 threads of a wave that share a step are an odd number of 64-byte rows apart and spread over all
-// LDS banks; with EU even, one spare row per group restores that.
+// thread jre of a column owns the EU pairs from u = jre*EU (the last few threads one more, each
+// shifted by one per thread before it that took one more).  Rows hold MP = NL + 1 values (8-byte
+// units), so the rows that consecutive threads read at one step are an odd number of units apart
+// (EU rows, plus one spare row per group of EU rows when EU is even) and fall on distinct LDS banks.
 template <int N, int EU> struct RowMap {
     static constexpr int PADR = (EU % 2 == 0) ? 1 : 0;
     static constexpr int GL = (N / 2) / EU + 1;   // groups of the lower half k <= N/2
@@ -320,8 +321,9 @@ template <int N, int EU> struct RowMap {
     }
 };
 
-// Last radix stage of the forward transform, written back to LDS in the RowMap order.
-template <typename R, int N, int NL, int T, class RM>
+// Last radix stage of the forward transform, written back to LDS in the RowMap order with MP
+// values per row.
+template <typename R, int N, int NL, int T, class RM, int MP>
 __device__ __forceinline__ void last_stage_mapped(cplx<R>* s, const cplx<R>* tw, int tid) {
     constexpr Radices PL = make_radices(N);
     constexpr int RAD = PL.r[PL.n - 1];
@@ -350,7 +352,7 @@ __device__ __forceinline__ void last_stage_mapped(cplx<R>* s, const cplx<R>* tw,
         if (NBF % T == 0 || g < NBF) {
             const int l = g % NL, j = g / NL;
 #pragma unroll
-            for (int m = 0; m < RAD; ++m) s[RM::row(j + m * NS) * NL + l] = v[b][m];
+            for (int m = 0; m < RAD; ++m) s[RM::row(j + m * NS) * MP + l] = v[b][m];
         }
     }
     __syncthreads();
@@ -386,8 +388,9 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
     static_assert(T % NL == 0, "bad binning geometry");
     using F = First<N, NL, T>;
     using RM = RowMap<N, (EU > 0 ? EU : 1)>;
+    constexpr int MP = NL + 1;               // values per row of the mapped layout
     constexpr bool ONE = F::NST == 1;
-    __shared__ cplx<R> s[(RM::ROWS > N ? RM::ROWS : N) * NL];
+    __shared__ cplx<R> s[(RM::ROWS * MP > N * NL ? RM::ROWS * MP : N * NL)];
     __shared__ cplx<R> tw[N];
     __shared__ R win[N];
     extern __shared__ double hs[];
@@ -417,7 +420,6 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
     const long long KEST = last == 0 ? (long long)N * W : W;  // the same in `keep`
     const long long KUST = last == 0 ? W : (long long)N * W;
     const int l = tid % NL;
-    const int jrow = tid / NL;
     const bool do1 = flags & PK_1D, do2 = flags & PK_2D, doph = flags & PK_PHASE;
     // column blocks of this block: items it = 0 .. nit-1
     int u0, kzb0, nit;
@@ -464,7 +466,7 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
                         const int jj = g / NL;
                         Dft<R, F::RAD>::run(cur[b]);
 #pragma unroll
-                        for (int m = 0; m < F::RAD; ++m) s[RM::row(jj * F::RAD + m) * NL + l] = cur[b][m];
+                        for (int m = 0; m < F::RAD; ++m) s[RM::row(jj * F::RAD + m) * MP + l] = cur[b][m];
                     }
                 }
                 __syncthreads();
@@ -474,13 +476,18 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
         }
         if constexpr (!ONE) {
             mid_stages<R, N, NL, NL, T, 1, 1, F::RAD>(s, tw, tid);
-            last_stage_mapped<R, N, NL, T, RM>(s, tw, tid);
+            last_stage_mapped<R, N, NL, T, RM, MP>(s, tw, tid);
         }
         int u, kzz;
         item(it, u, kzz);
         // the first kz of the column block (the same for the whole block): only the blocks that hold
         // kz = 0 or kz = N/2 apply the independent-mode rule of those planes (Pk_library.pyx:322-330)
         const int kzlo = kzz - l;
+        // the epilogue takes its own lane layout: TPC consecutive threads share a column (le) and hold
+        // consecutive runs of row pairs (jre), so that the threads of a wave flush different bins
+        // (with consecutive columns per wave they mostly share one, and their LDS atomics serialise)
+        const int le = tid / TPC, jre = tid % TPC;
+        kzz = kzlo + le;
         const bool zblock = kzlo == 0 || (even && middle >= kzlo && middle < kzlo + NL);
         if (kzz >= W || (drop0 && kzz == 0)) continue;
         const int kz = kzz;
@@ -496,12 +503,12 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
         // |k|^2 = c0 + u^2 and k_perp^2 = cper + u^2 of this thread's first pair and their bins
         // floor(sqrt); each step of u adds at most 1 to sqrt(c + u^2) (its derivative u / sqrt(c + u^2)
         // is <= 1), so each bin grows by 0 or 1 per step and is carried along without a square root
-        const int extra = jrow - (TPC - UR);   // >= 0: this thread takes EU + 1 pairs
-        const int uq0 = jrow * EU + (extra > 0 ? extra : 0);
+        const int extra = jre - (TPC - UR);   // >= 0: this thread takes EU + 1 pairs
+        const int uq0 = jre * EU + (extra > 0 ? extra : 0);
         const int ne = EU + (extra >= 0 ? 1 : 0);
         int k2 = c0 + uq0 * uq0, kp2 = cper + uq0 * uq0;
         int bin = isqrt24(k2), wb = isqrt24(kp2);
-        const int w2o = fixk ? l * nper : 0;
+        const int w2o = fixk ? le * nper : 0;
         // the current runs: 3-D bin cur3 and 2-D entry cur2 with their sums and modes; a run is added
         // to the LDS histograms when the bin changes and at the end (if it holds modes)
         int cur3 = bin, cur2 = w2o + wb;
@@ -522,8 +529,8 @@ __device__ __forceinline__ void last_bin_body(const LastBinArgs<R>& A, Blk B) {
                 }
                 const int ja = uu, jb = (N - uu) % N;
                 const bool pair = jb != ja;
-                const cplx<R> va = s[RM::row(ja) * NL + l];
-                const cplx<R> vb = pair ? s[RM::row(jb) * NL + l] : mk<R>(0, 0);
+                const cplx<R> va = s[RM::row(ja) * MP + le];
+                const cplx<R> vb = pair ? s[RM::row(jb) * MP + le] : mk<R>(0, 0);
                 bool ia = true, ib = pair;
                 if (ZP && zplane) {
                     const int xa = last == 0 ? uu : ku, ya = last == 0 ? ku : uu;
