@@ -71,8 +71,10 @@ const char* pyl_last_error(void);
 /* ---------------------------------------------------------------- device memory
  * No counterpart in the reference (its arrays live in host memory): the few device-memory calls a
  * caller without a device-array library needs to use the entry points below from host arrays (the
- * numpy drop-in path).  pyl_device_count reports 0 without a GPU or driver; pyl_memcpy_h2d,
- * pyl_memcpy_d2d and pyl_dev_zero are asynchronous on `stream` (NULL: the default stream);
+ * numpy drop-in path).  pyl_device_count reports 0 without a GPU or driver.  pyl_memcpy_h2d returns
+ * once `src` may be reused: the data goes through the library's pinned staging buffers (3 x 32 MiB,
+ * filled by host threads while the DMA of the previous chunk runs) and its copies are queued on
+ * `stream` (NULL: the default stream).  pyl_memcpy_d2d and pyl_dev_zero are asynchronous on `stream`;
  * pyl_memcpy_d2h returns when the copy is done. */
 int pyl_device_count(int* count);
 int pyl_dev_alloc(void** ptr, int64_t bytes);

@@ -266,7 +266,8 @@ class DevArray:
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
         if self.nbytes:
-            check(lib().pyl_memcpy_d2h(out.ctypes.data_as(_c.c_void_p), _c.c_void_p(self._ptr), self.nbytes, None))
+            check(lib().pyl_memcpy_d2h(out.ctypes.data_as(_c.c_void_p), _c.c_void_p(self._ptr), self.nbytes,
+                                       stream()))
         return out
 
     def __del__(self):
@@ -298,12 +299,10 @@ def to_dev(a, dtype):
     if isinstance(a, DevArray):
         return a if a.dtype == np_dtype(dtype) else to_dev(a.numpy(), dtype)
     arr = np.ascontiguousarray(a, dtype=np_dtype(dtype))
-    if torch_mode():
-        return _torch().from_numpy(arr).cuda()
-    d = DevArray(arr.shape, arr.dtype)
-    if d.nbytes:
-        check(lib().pyl_memcpy_h2d(_c.c_void_p(d.data_ptr()), arr.ctypes.data_as(_c.c_void_p), d.nbytes, None))
-        check(lib().pyl_stream_sync(None))
+    d = empty(arr.shape, arr.dtype)
+    if arr.nbytes:
+        # staged through the library's pinned buffers; returns once `arr` may be freed
+        check(lib().pyl_memcpy_h2d(ptr(d), arr.ctypes.data_as(_c.c_void_p), arr.nbytes, stream()))
     return d
 
 
@@ -329,19 +328,26 @@ def copy_rows(dev, first, host):
     arr = np.ascontiguousarray(host, dtype=dtype_of(dev))
     if arr.shape[0] == 0:
         return
-    if is_tensor(dev):
+    if is_tensor(dev) and not dev.is_contiguous():
         dev[first:first + arr.shape[0]].copy_(_torch().from_numpy(arr))
         return
-    row = dev.nbytes // dev.shape[0]
+    row = (dev.element_size() * dev.numel() if is_tensor(dev) else dev.nbytes) // dev.shape[0]
     check(lib().pyl_memcpy_h2d(_c.c_void_p(dev.data_ptr() + first * row), arr.ctypes.data_as(_c.c_void_p),
-                               arr.nbytes, None))
-    check(lib().pyl_stream_sync(None))
+                               arr.nbytes, stream()))
 
 
 def to_host(t):
     if isinstance(t, DevArray):
         return t.numpy()
-    return t.detach().cpu().numpy()
+    t = t.detach()
+    if not t.is_cuda:
+        return t.numpy()
+    t = t.contiguous()
+    out = np.empty(tuple(t.shape), np_dtype(t.dtype))
+    if out.nbytes:
+        # staged through the library's pinned buffers, after the work queued on torch's stream
+        check(lib().pyl_memcpy_d2h(out.ctypes.data_as(_c.c_void_p), ptr(t), out.nbytes, stream()))
+    return out
 
 
 def profile(enable):
