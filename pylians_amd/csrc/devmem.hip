@@ -59,7 +59,8 @@ void host_copy(char* dst, const char* src, size_t n, int threads) {
         memcpy(dst, src, n);
         return;
     }
-    const size_t part = (n / threads + 4095) & ~(size_t)4095;
+    // parts of a whole number of pages, threads * part >= n
+    const size_t part = ((n + threads - 1) / threads + 4095) & ~(size_t)4095;
     std::vector<std::thread> pool;
     for (int t = 1; t < threads; ++t) {
         const size_t a = std::min(n, part * t), b = std::min(n, part * (t + 1));
