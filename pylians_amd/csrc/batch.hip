@@ -272,7 +272,7 @@ static int batch_build(const pyl_pk_job* jobs, int njobs, hipStream_t st, BatchP
         if (rc) return rc;
         B->zero.add(ZeroArgs{J.bj.cnt, J.bj.cnt_bytes}, 0);
         if (J.bj.scnt) B->zero.add(ZeroArgs{J.bj.scnt, J.bj.scnt_bytes}, 0);
-        const int acc = f64 ? 0 : acc_mode(j.flags);
+        const int acc = dense_acc(f64 ? 0 : acc_mode(j.flags), j.scheme, j.n, j.dims);
         auto key = std::make_tuple(j.scheme, j.prec, acc, j.W ? 1 : 0, J.kind == DK_SORTED ? 1 : 0);
         DepClass& D = B->dep[key];
         D.scheme = j.scheme;

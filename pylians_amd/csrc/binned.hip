@@ -31,6 +31,9 @@ constexpr int BT = 512;    // k_bin: threads per workgroup
 constexpr int KP = 4;      // k_bin: particles per thread per step
 constexpr int HT = 4096;   // k_bin: LDS hash slots (> BT * KP, so an insertion always finds a slot)
 constexpr int TT = 512;    // k_tile: threads per workgroup
+// particles per cell from which CIC tiles accumulate in register runs (measured at 512^3 particles: 8 per
+// cell 0.94 -> 0.78 ms, 4.1: 0.82 -> 0.75, 2.4: equal, 1.5: 0.75 -> 0.81; NGP gains nothing)
+constexpr double RUN_PPC = 3.0;
 // row stride of the float64 LDS accumulator of a tile: TILE + 1 doubles, so that particles one row
 // apart in y (neighbours in the input along z fill a row, then the next) land on LDS banks 34 apart
 // (measured on the 512^3 lattice: k_tile 0.88 -> 0.77 ms; strides 18/20/24 and padded planes were slower)
@@ -441,12 +444,17 @@ __device__ __forceinline__ void accumulate32(double* acc, const float* r, int N,
     }
 }
 
-// ACC: ACC_F64 (float64 weights, float64 cell sums) or ACC_F32W (float32 weights and products,
-// float64 cell sums).  Both run at the same speed on gfx950 (the kernel is bound by loads and LDS
+// ACC: ACC_F64 (float64 weights, float64 cell sums), ACC_RUN (the same, with register runs of the
+// stencil block for dense CIC inputs) or ACC_F32W (float32 weights and products, float64
+// cell sums).  Both run at the same speed on gfx950 (the kernel is bound by loads and LDS
 // atomics, not by float64 arithmetic); a float32 LDS cell (ds_add_f32) was measured about six times
 // slower than the float64 one (ds_add_f64) in this kernel, and a TwoSum-compensated float32 cell
 // (returning atomics) twelve times slower, so the float64 cell is the accumulator of every mode.
-enum { ACC_F64 = 0, ACC_F32W = 1 };
+enum { ACC_F64 = 0, ACC_F32W = 1, ACC_RUN = 2 };
+
+int dense_acc(int acc, int scheme, int64_t n, int dims) {
+    return acc == ACC_F64 && scheme == 2 && (double)n >= RUN_PPC * (double)dims * dims * dims ? ACC_RUN : acc;
+}
 
 template <int S, typename G, bool HASW, int MODE, bool UNIT, int ACC>
 __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
@@ -540,8 +548,93 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, Blk B) {
     // phase 1: the tile's own class-0 records (most of them): one contiguous segment whose stencils
     // stay inside the tile -- no entry search, no clipping, no overflow class word
     const unsigned n0 = scnt[0];
-    {
-        const float* q0 = a.bucket + sbase[0] * R;
+    const float* q0 = a.bucket + sbase[0] * R;
+    if constexpr (S == 2 && ACC == ACC_RUN) {
+        // Runs (dense inputs): a thread takes groups of RG consecutive records (consecutive particles
+        // of the input, mostly neighbours along z) and keeps the float64 sums of its current S^3
+        // stencil block in registers.  A record with the same lowest cell adds to them; one cell
+        // further along z, the block's lower z face goes to LDS and the block moves up (CIC: 4 LDS
+        // atomics instead of 8); any other record sends the whole block to LDS.  With several
+        // particles per cell most records add in registers (512^3 particles onto 256^3: 0.95 ->
+        // 0.79 ms); with about one per cell the strided group loads and the register blocks cost more
+        // than the atomics they save, so the host picks this mode by particles per cell.
+        constexpr int RG = 2;
+        double blk[S][S][S];
+        int b0 = -1, b1 = 0, b2 = 0;   // lowest cell of the block (b0 < 0: none yet)
+        float gr[RG][4], gn[RG][4];
+        auto fetchg = [&](unsigned g) {
+#pragma unroll
+            for (int k = 0; k < RG; ++k) {
+                const unsigned fk = g * RG + k;
+                const float* q = q0 + (size_t)(fk < n0 ? fk : 0u) * R;   // record 0 of the segment always exists
+#pragma unroll
+                for (int c = 0; c < R; ++c) gn[k][c] = q[c];
+            }
+        };
+        auto all_out = [&]() {
+#pragma unroll
+            for (int l = 0; l < S; ++l)
+#pragma unroll
+                for (int m = 0; m < S; ++m) {
+                    double* row = acc + ((b0 + l) * TILE + b1 + m) * TROW + b2;
+#pragma unroll
+                    for (int q = 0; q < S; ++q) atomicAdd(row + q, blk[l][m][q]);
+                }
+        };
+        const unsigned ngr = (n0 + RG - 1) / RG;
+        fetchg(tid);
+        for (unsigned g = tid; g < ngr; g += TT) {
+#pragma unroll
+            for (int k = 0; k < RG; ++k)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) gr[k][c] = gn[k][c];
+            fetchg(g + TT);
+#pragma unroll
+            for (int k = 0; k < RG; ++k) {
+                if (g * RG + k >= n0) break;
+                double w0[S], w1[S], w2[S];
+                int l0 = wrapN(Axis<S>::eval((double)gr[k][0] * a.inv, w0), a.N) - ox;
+                int l1 = wrapN(Axis<S>::eval((double)gr[k][1] * a.inv, w1), a.N) - oy;
+                int l2 = wrapN(Axis<S>::eval((double)gr[k][2] * a.inv, w2), a.N) - oz;
+                if (l0 >= TILE) l0 -= a.N;
+                if (l1 >= TILE) l1 -= a.N;
+                if (l2 >= TILE) l2 -= a.N;
+                if (l0 != b0 || l1 != b1 || l2 != b2) {
+                    if (S == 2 && l0 == b0 && l1 == b1 && l2 == b2 + 1) {
+#pragma unroll
+                        for (int l = 0; l < S; ++l)
+#pragma unroll
+                            for (int m = 0; m < S; ++m) {
+                                atomicAdd(acc + ((b0 + l) * TILE + b1 + m) * TROW + b2, blk[l][m][0]);
+                                blk[l][m][0] = blk[l][m][S - 1];
+                                blk[l][m][S - 1] = 0.0;
+                            }
+                    } else {
+                        if (b0 >= 0) all_out();
+#pragma unroll
+                        for (int l = 0; l < S; ++l)
+#pragma unroll
+                            for (int m = 0; m < S; ++m)
+#pragma unroll
+                                for (int q = 0; q < S; ++q) blk[l][m][q] = 0.0;
+                        b0 = l0;
+                        b1 = l1;
+                    }
+                    b2 = l2;
+                }
+                const double ww = HASW ? (double)gr[k][3] * a.mult : a.mult;
+#pragma unroll
+                for (int l = 0; l < S; ++l)
+#pragma unroll
+                    for (int m = 0; m < S; ++m) {
+                        const double wlm = UNIT ? w0[l] * w1[m] : w0[l] * w1[m] * ww;
+#pragma unroll
+                        for (int q = 0; q < S; ++q) blk[l][m][q] += wlm * w2[q];
+                    }
+            }
+        }
+        if (b0 >= 0) all_out();
+    } else {
         auto fetch0 = [&](unsigned f) {
 #pragma unroll
             for (int j = 0; j < RP; ++j) {
@@ -840,7 +933,13 @@ static int run_binned(const float* pos, const float* W, int64_t n, int dims, dou
         const dim3 tg((unsigned)((ntiles + XCDS - 1) / XCDS * XCDS));
         const bool unit = !HASW && mult == 1.0;
 #define PYL_TILE_LAUNCH(U, A) hipLaunchKernelGGL((k_tile<S, G, HASW, MODE, U, A>), tg, dim3(TT), 0, st, ta)
-        if (acc == ACC_F32W) {
+        if constexpr (S == 2) {
+            if (acc == ACC_RUN) {
+                if (unit) PYL_TILE_LAUNCH(true, ACC_RUN); else PYL_TILE_LAUNCH(false, ACC_RUN);
+            }
+        }
+        if (acc == ACC_RUN) {
+        } else if (acc == ACC_F32W) {
             if (unit) PYL_TILE_LAUNCH(true, ACC_F32W); else PYL_TILE_LAUNCH(false, ACC_F32W);
         } else {
             if (unit) PYL_TILE_LAUNCH(true, ACC_F64); else PYL_TILE_LAUNCH(false, ACC_F64);
@@ -936,10 +1035,12 @@ int deposit_binned(const float* pos, const float* W, int64_t n, int dims, double
         rsd = Rsd();
     }
     if (f64) {
-        if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, ACC_F64);
-        if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, ACC_F64);
-        return dispatch_s<double, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, ACC_F64);
+        const int a64 = dense_acc(ACC_F64, scheme, n, dims);
+        if (mode == OUT_ACC) return dispatch_s<double, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, a64);
+        if (mode == OUT_SET) return dispatch_s<double, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, a64);
+        return dispatch_s<double, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, a64);
     }
+    acc = dense_acc(acc, scheme, n, dims);
     if (mode == OUT_ACC) return dispatch_s<float, OUT_ACC>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
     if (mode == OUT_SET) return dispatch_s<float, OUT_SET>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
     return dispatch_s<float, OUT_DELTA>(pos, W, n, dims, BoxSize, scheme, mult, grid, dmass, mass, st, rsd, acc);
@@ -1064,6 +1165,13 @@ static int batch_launch_s(bool sorted, const BinnedTables& t, hipStream_t st) {
 
 template <int S>
 static int batch_launch_w(bool hasw, bool f64, int acc, bool sorted, const BinnedTables& t, hipStream_t st) {
+    if constexpr (S == 2) {
+        if (acc == ACC_RUN)
+            return f64 ? (hasw ? batch_launch_s<S, true, double, ACC_RUN>(sorted, t, st)
+                               : batch_launch_s<S, false, double, ACC_RUN>(sorted, t, st))
+                       : (hasw ? batch_launch_s<S, true, float, ACC_RUN>(sorted, t, st)
+                               : batch_launch_s<S, false, float, ACC_RUN>(sorted, t, st));
+    }
     if (f64) return hasw ? batch_launch_s<S, true, double, ACC_F64>(sorted, t, st)
                          : batch_launch_s<S, false, double, ACC_F64>(sorted, t, st);
     if (acc == ACC_F32W) return hasw ? batch_launch_s<S, true, float, ACC_F32W>(sorted, t, st)

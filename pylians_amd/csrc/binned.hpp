@@ -105,5 +105,9 @@ struct BinnedTables {
     unsigned tile_total;
 };
 int binned_batch_launch(int scheme, bool hasw, bool f64, int acc, bool sorted, const BinnedTables& t, hipStream_t st);
+// accumulation mode of a binned deposit: `acc` (0: float64 weights, 1: float32 weights), or 2 (float64
+// weights with register runs of the stencil block) for CIC inputs with at least RUN_PPC
+// particles per cell
+int dense_acc(int acc, int scheme, int64_t n, int dims);
 
 }  // namespace pyl

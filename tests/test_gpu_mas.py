@@ -253,3 +253,30 @@ def test_presorted_tiny_inputs(gpu):
         o = C.deposit_f64(pos, dims, L, "TSC")
         g = _dev_deposit(pos, None, dims, L, 3, _lib.SORTED | _lib.ZERO)
         assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, n
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_dense_cic_register_runs(gpu, ordered, weighted):
+    """CIC with at least 3 particles per cell takes register runs of the stencil block in the tiles
+    (MAS_library.pyx:123-166): lattice-ordered particles (most records extend a run) and unordered
+    ones (every record starts one), plain and presorted, overwrite and accumulate, float32 and float64
+    grids -- each cell the single rounding of its float64 sum, as the default accumulation."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(43)
+    dims, L = 48, 120.0
+    if ordered:
+        pos = synth.zeldovich(96, L, seed=12, rms_disp=2.0, sigma_v=50.0)      # 8 particles per cell
+    else:
+        pos = rng.random((5 * dims ** 3, 3), dtype=np.float32) * np.float32(L)  # 5 per cell
+    W = (rng.random(len(pos), dtype=np.float32) + np.float32(0.1)) if weighted else None
+    assert len(pos) >= 3 * dims ** 3
+    o = C.deposit_f64(pos, dims, L, "CIC", W)
+    for flags in (_lib.BINNED, _lib.SORTED):
+        g = _dev_deposit(pos, W, dims, L, 2, flags | _lib.ZERO)
+        assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, flags
+    start = rng.random((dims,) * 3).astype(np.float32)
+    gacc = _dev_deposit(pos, W, dims, L, 2, _lib.BINNED, start=start)
+    assert np.max(np.abs(gacc - (start + o)) / (start + o)) <= 6.5e-8
+    g64 = _dev_deposit(pos, W, dims, L, 2, _lib.BINNED | _lib.ZERO, f64=True)
+    assert np.allclose(g64, o, rtol=1e-13, atol=1e-13)
