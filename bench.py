@@ -90,6 +90,16 @@ def bk_bytes(n_p, N, ntheta):
     return 12 * n_p + 4 * Ng + (4 * Ng + 8 * Nh) + (ntheta + 2) * (8 * Nh + 4 * Ng) + ntheta * 12 * Ng
 
 
+def bk_bytes_parseval(n_p, N, K):
+    """The compulsory traffic of this implementation's Bk (indicator sums cached): the deposit
+    (12 N_p + 4 N_g), the r2c of delta (4 N_g + 8 N_h), the c2r of the k1 and k2 shells (8 N_h + 4 N_g
+    each, the second writing d1 d2), the r2c of d1 d2 (4 N_g + 8 N_h) and one read of the two spectra's
+    columns kz < K by the theta-shell mode sums (2 x 8 N^2 K)."""
+    Ng = N ** 3
+    Nh = N * N * (N // 2 + 1)
+    return 12 * n_p + 4 * Ng + 2 * (4 * Ng + 8 * Nh) + 2 * (8 * Nh + 4 * Ng) + 16 * N * N * K
+
+
 def cpu_leg(pos_host, N, L, axis, gpu_pk, threads):
     """Reference CPU path timed on this host + FP64 oracle error of the GPU result (checker only)."""
     from oracle import cpu as C
@@ -226,10 +236,18 @@ def workload_bispectrum(args):
     ms = (time.perf_counter() - t0) * 1e3 / len(parts)
     n = int(parts[0].shape[0])
     nbytes = bk_bytes(n, N, nth)
+    # the theta shells reach |k| < k1 + k2 + kF (kF units): columns kz < K of the spectra
+    kF = 2.0 * np.pi / L
+    K = min(N // 2 + 1, int(np.floor((0.1 + 0.2 + kF) / kF)) + 2)
+    own = bk_bytes_parseval(n, N, K)
     del parts
     return {"metric": "Bk per realization (512^3 particles CIC onto 256^3, 20 theta bins, indicator sums cached)",
             "value": round(ms, 3), "unit": "ms/realization", "higher_is_better": False, "dtype": "f32",
             "bytes": nbytes, "e2e_hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "bytes_parseval": own, "hbm_frac_parseval": round(own / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "note": "bytes: the estimator's 22 shell c2r and 20 triple products (the reference algorithm); this "
+                    "build sums the theta shells over their modes by Parseval after one r2c of d1 d2, and "
+                    "bytes_parseval is its own compulsory traffic",
             "config": {"workload": "bk_zeldovich512_grid256_20theta", "realizations": args.realizations}}
 
 

@@ -3,10 +3,13 @@
 // delta_k = r2c(delta); for every shell i (k1, k2, k3(theta_j)) the masked spectra
 //   D_i(k) = W(k) delta_k and I_i(k) = 1 on kmin_i <= |k| < kmax_i (kF units, every half-spectrum
 //   mode: the independence cut is commented out in the reference, :100-105)
-// are inverse transformed with 1/N^3 normalisation.  The shell mask and the window are applied
-// inside the first inverse pass (no masked spectrum is materialised) and the float64 reductions
-// sum d_i^2, sum I_i^2, sum d1 d2 d3, sum I1 I2 I3 are fused into the last (c2r rows) pass, which
-// multiplies by the stored d1*d2 (resp. I1*I2) field instead of writing d3.
+// are inverse transformed with 1/N^3 normalisation (shells k1 and k2: the shell mask and the window
+// are applied inside the first inverse pass, and the float64 reductions sum d_i^2, sum I_i^2 and the
+// products d1*d2, I1*I2 come out of the last, c2r rows, pass).  The theta shells k3 need sums only:
+// by Parseval, sum_x d3 p = (1/N^3) sum_k D3(k) conj(P(k)) over the full spectrum with P = FFT(p),
+// p = d1 d2 (or I1 I2), and sum_x d3^2 = (1/N^3) sum_k |D3(k)|^2.  So p is transformed forward once
+// and every theta shell is a sum over its modes; the modes are binned by their integer |k|^2 = m and
+// a shell kmin <= sqrt(m) < kmax is a sum of bins, with the membership test of the inverse path.
 #include <math.h>
 
 #include <algorithm>
@@ -65,6 +68,34 @@ __global__ __launch_bounds__(T) void k_shell_x_inv(const cplx<R>* __restrict__ s
     lds_fft<R, N, NL, NL, T, 1>(s, tw, tid);
     if (valid)
         for (int j = tid / NL; j < N; j += T / NL) dst[c + j * ncols] = conjg(s[j * NL + l]);
+}
+
+// Mode sums of the theta shells, binned by m = |k|^2 in [mlo, mhi) (modes with kz < K): per bin
+// sum W delta conj(P), sum |W delta|^2, sum conj(PI) (pi may be null) and the mode count, each mode of
+// the half spectrum weighted 2 (0 < kz < N/2: its mirror -k adds the complex conjugate) or 1 (the
+// kz = 0 and N/2 planes, which hold both k and -k).  Sums go to `bins` with float64 atomics.
+template <typename R>
+__global__ __launch_bounds__(256) void k_bk_modes(const cplx<R>* __restrict__ spec, const cplx<R>* __restrict__ pd,
+                                                   const cplx<R>* __restrict__ pi, long long P, int N, int K, int mlo,
+                                                   int mhi, const double* __restrict__ win, double* __restrict__ bins) {
+    const long long total = (long long)N * N * K;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const int kz = (int)(i % K);
+        const int kyy = (int)((i / K) % N), kxx = (int)(i / ((long long)K * N));
+        const int kx = kfreq(kxx, N), ky = kfreq(kyy, N);
+        const int m = kx * kx + ky * ky + kz * kz;
+        if (m < mlo || m >= mhi) continue;
+        const long long c = ((long long)kxx * N + kyy) * P + kz;
+        const double wz = (kz == 0 || 2 * kz == N) ? 1.0 : 2.0;
+        const double w = win[kxx] * win[kyy] * win[kz];
+        const cplx<R> d = spec[c], q = pd[c];
+        const double dx = w * (double)d.x, dy = w * (double)d.y;
+        double* b = bins + 4 * (long long)(m - mlo);
+        unsafeAtomicAdd(b + 0, wz * (dx * (double)q.x + dy * (double)q.y));
+        unsafeAtomicAdd(b + 1, wz * (dx * dx + dy * dy));
+        if (pi) unsafeAtomicAdd(b + 2, wz * (double)pi[c].x);
+        unsafeAtomicAdd(b + 3, wz);
+    }
 }
 
 template <typename R, int N>
@@ -186,15 +217,59 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
         if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[0], kmax[0], 1, win, I1, 1, nullptr, acc + 2, st))) return rc;
         if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
     }
+    // theta shells: P = FFT(d1 d2) (and FFT(I1 I2)), then the mode sums binned by |k|^2
+    double lo = 1e300, hi = 0.0;
     for (int j = 0; j < nb; ++j) {
-        double* a = acc + 4 * (j + 2);
-        if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[j + 2], kmax[j + 2], 0, win, nullptr, 0, d12, a, st))) return rc;
-        if (need_i && (rc = shell_field<R>(spec, ws, N, P, K, kmin[j + 2], kmax[j + 2], 1, win, nullptr, 0, I12, a + 2, st)))
-            return rc;
+        lo = std::min(lo, kmin[j + 2]);
+        hi = std::max(hi, kmax[j + 2]);
+    }
+    const int mlo = nb ? (int)std::floor(std::max(lo, 0.0)) * (int)std::floor(std::max(lo, 0.0)) : 0;
+    const int mhi = nb ? std::max(mlo, (int)std::ceil(std::max(hi, 0.0)) * (int)std::ceil(std::max(hi, 0.0)) + 1) : 0;
+    const size_t nbin = (size_t)(mhi - mlo);
+    std::vector<double> hb(4 * nbin);
+    if (nbin) {
+        cplx<R>* pd = ws;   // the shell transforms are done with it
+        cplx<R>* pi = nullptr;
+        if ((rc = fft3d_r2c<R>(d12, pd, N, P, st))) return rc;
+        if (need_i) {
+            pi = (cplx<R>*)workspace(WS_AUX3, sizeof(cplx<R>) * nspec, &rc);
+            if (rc) return rc;
+            if ((rc = fft3d_r2c<R>(I12, pi, N, P, st))) return rc;
+        }
+        double* bins = (double*)workspace(WS_BINS, sizeof(double) * 4 * nbin, &rc);
+        if (rc) return rc;
+        if ((rc = zero_fill(bins, sizeof(double) * 4 * nbin, st))) return rc;
+        const long long total = (long long)N * N * K;
+        const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_bk_modes<R>, dim3(blocks), dim3(256), 0, st, spec, pd, pi, (long long)P, N, K, mlo, mhi,
+                           win, bins);
+        PYL_HIP(hipGetLastError());
+        PYL_HIP(hipMemcpyAsync(hb.data(), bins, sizeof(double) * 4 * nbin, hipMemcpyDeviceToHost, st));
     }
     std::vector<double> h(nacc);
     PYL_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st));
     PYL_HIP(hipStreamSynchronize(st));
+    // shell j: the bins kmin <= sqrt(m) < kmax; by Parseval each sum carries 1/N^3
+    const double inv3 = 1.0 / ((double)N * N * N);
+    for (int j = 0; j < nb; ++j) {
+        double* a = h.data() + 4 * (j + 2);
+        double sdp = 0.0, sdd = 0.0, sip = 0.0, sii = 0.0;
+        for (int m = mlo; m < mhi; ++m) {
+            const double k = sqrt((double)m);
+            if (!(k >= kmin[j + 2] && k < kmax[j + 2])) continue;
+            const double* b = hb.data() + 4 * (size_t)(m - mlo);
+            sdp += b[0];
+            sdd += b[1];
+            sip += b[2];
+            sii += b[3];
+        }
+        a[0] = sdd * inv3;
+        a[1] = sdp * inv3;
+        if (need_i) {
+            a[2] = sii * inv3;
+            a[3] = sip * inv3;
+        }
+    }
     if (need_i) {
         isums.resize(2 * (size_t)(nb + 2));
         for (int i = 0; i < nb + 2; ++i) {
