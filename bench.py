@@ -209,6 +209,27 @@ def workload_adjoint(args):
             "config": {"workload": "zeldovich256_cic_adjoint_64bins", "n_particles": n, "grid": N, "BoxSize": L}}
 
 
+def workload_xpk(args):
+    """XPk of two 512^3 float32 fields resident in HBM (L = 1000, axis 0, CIC windows): both auto and the
+    cross spectrum with every output (Pk_library.pyx:534-798), one whole call each."""
+    import torch
+
+    from pylians_amd import Pk_library
+    N, L = 512, 1000.0
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    a = torch.randn((N, N, N), device="cuda", dtype=torch.float32, generator=g)
+    b = (0.5 * a + torch.randn((N, N, N), device="cuda", dtype=torch.float32, generator=g)).contiguous()
+    ms = _timed(lambda: Pk_library.XPk([a, b], L, axis=0, MAS=["CIC", "CIC"]), args.steps, args.warmup)
+    Ng, Nh = N ** 3, N * N * (N // 2 + 1)
+    nbytes = 2 * (4 * Ng + 8 * Nh + 8 * Nh)   # per field: r2c (read grid, write spectrum), binning read
+    del a, b
+    return {"metric": "XPk of two 512^3 fields (auto + cross, all outputs)", "value": round(ms, 3),
+            "unit": "ms/call", "ms_per_step": round(ms, 4), "higher_is_better": False, "dtype": "f32",
+            "bytes": nbytes, "e2e_hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "config": {"workload": "xpk_two_fields_grid512", "fields": 2, "grid": N, "BoxSize": L}}
+
+
 def workload_bispectrum(args):
     """Realizations of 512^3 Zel'dovich particles (resident in HBM), each: CIC onto a 256^3 grid,
     delta = rho/mean - 1, Bk(k1=0.1, k2=0.2, 20 theta) -- the deposit is inside the timed region."""
@@ -265,7 +286,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="before the timed region, copy a 512 MiB device buffer twice (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--workload", default="zeldovich512", choices=["zeldovich512", "batch", "adjoint", "bispectrum"])
+    ap.add_argument("--workload", default="zeldovich512", choices=["zeldovich512", "batch", "adjoint", "bispectrum",
+                                                                   "xpk"])
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--realizations", type=int, default=16)
     ap.add_argument("--no-extra", action="store_true",
@@ -278,7 +300,8 @@ def main():
         print("bench.py measures one MI355X: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "zeldovich512":
-        fn = {"batch": workload_batch, "adjoint": workload_adjoint, "bispectrum": workload_bispectrum}[args.workload]
+        fn = {"batch": workload_batch, "adjoint": workload_adjoint, "bispectrum": workload_bispectrum,
+              "xpk": workload_xpk}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 
@@ -413,11 +436,11 @@ def main():
         ex.steps, ex.warmup, ex.realizations = 5, 2, 4
         extra = {}
         for key, fn in (("batch64", workload_batch), ("adjoint256", workload_adjoint),
-                        ("bispectrum256", workload_bispectrum)):
+                        ("bispectrum256", workload_bispectrum), ("xpk512", workload_xpk)):
             t1 = time.time()
             r = fn(ex)
-            extra[key] = {k: r[k] for k in ("value", "unit", "ms_per_step", "bytes", "e2e_hbm_frac", "config")
-                          if k in r}
+            extra[key] = {k: r[k] for k in ("value", "unit", "ms_per_step", "bytes", "e2e_hbm_frac", "bytes_parseval",
+                                            "hbm_frac_parseval", "note", "config") if k in r}
             print("# %s %.1fs" % (key, time.time() - t1), file=sys.stderr, flush=True)
             torch.cuda.empty_cache()
         line["workloads"] = extra

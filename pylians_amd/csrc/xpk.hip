@@ -102,11 +102,25 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
             const long long base = (long long)u * UST + kz;
             const int ku = kfreq(u, N);
             const bool zplane = (kz == 0) || (kz == middle && even);
-            // window factors of the two fixed coordinates (u and kz), per field
+            // product of the window factors of the two fixed coordinates (u and kz), per field; a mode's
+            // window is this times the factor of its k_j (Pk_library.pyx:688-697, in float64, rounded
+            // once to the working precision)
             double wuz[NF];
 #pragma unroll
             for (int f = 0; f < NF; ++f)
-                if (f < nf) wuz[f] = wp[f][u];
+                if (f < nf) wuz[f] = wp[f][u] * wp[f][kz];
+            // |k|^2, k_perp^2 of the thread's first mode in the column and their bins floor(sqrt); along
+            // the run |k_j| changes by one per step (across j = N/2 too), so each bin moves by at most one
+            int bin = 0, wb = 0;
+            {
+                const int j0 = jrow * E;
+                if (j0 < N) {
+                    const int kj = kfreq(j0, N);
+                    const int kx = a.last == 0 ? kj : ku, ky = a.last == 0 ? ku : kj;
+                    bin = isqrt24(kx * kx + ky * ky + kz * kz);
+                    wb = isqrt24(a.axis == 0 ? ky * ky + kz * kz : (a.axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
+                }
+            }
             int cur3 = -1, cur2 = -1;
             unsigned m3 = 0u, m1 = 0u, m2 = 0u;   // modes of the current runs
             double acc[NC][3], b1[NC], b2[NC];
@@ -139,24 +153,22 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
                     const int kj = kfreq(j, N);
                     const int kx = a.last == 0 ? kj : ku;
                     const int ky = a.last == 0 ? ku : kj;
-                    if (zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0))) continue;
                     const int k2 = kx * kx + ky * ky + kz * kz;
-                    const int bin = isqrt24(k2);   // |k|^2 <= 3 (N/2)^2 < 2^24 on every supported grid
+                    const int kp2 = a.axis == 0 ? ky * ky + kz * kz : (a.axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky);
+                    bin += __mul24(bin + 1, bin + 1) <= k2 ? 1 : 0;
+                    bin -= __mul24(bin, bin) > k2 ? 1 : 0;
+                    wb += __mul24(wb + 1, wb + 1) <= kp2 ? 1 : 0;
+                    wb -= __mul24(wb, wb) > kp2 ? 1 : 0;
+                    if (zplane && (kx < 0 || ((kx == 0 || (kx == middle && even)) && ky < 0))) continue;
                     const int kps = a.axis == 0 ? kx : (a.axis == 1 ? ky : kz);
                     const R mu2 = k2 == 0 ? (R)0 : (R)(kps * kps) * recip((R)k2);
                     const R L2 = ((R)3 * mu2 - (R)1) * (R)0.5;
                     const R L4 = (((R)35 * mu2 - (R)30) * mu2 + (R)3) * (R)0.125;
-                    // each field's value times its window W(x) W(y) W(z), the product taken in the
-                    // order wa[x] * wa[y] * wa[z] (Pk_library.pyx:688-697)
+                    // each field's value times its window W(x) W(y) W(z)
                     cplx<R> w[NF];
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
-                        if (f < nf) {
-                            const double* wf = wp[f];
-                            const double wx = a.last == 0 ? wj[b][f] : wuz[f];
-                            const double wy = a.last == 0 ? wuz[f] : wj[b][f];
-                            w[f] = scale(v[b][f], (R)(wx * wy * wf[kz]));
-                        }
+                        if (f < nf) w[f] = scale(v[b][f], (R)(wj[b][f] * wuz[f]));
                     R val[NC];
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
@@ -192,8 +204,7 @@ __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
 #pragma unroll
                         for (int c = 0; c < NC; ++c) b1[c] += (double)val[c];
                     }
-                    const int kper = isqrt24(a.axis == 0 ? ky * ky + kz * kz
-                                                         : (a.axis == 1 ? kx * kx + kz * kz : kx * kx + ky * ky));
+                    const int kper = wb;
                     const int w2 = (a.fixk ? l * nper : 0) + kper;
                     if (w2 != cur2) {
                         if (cur2 >= 0) {
