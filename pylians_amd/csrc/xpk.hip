@@ -295,10 +295,9 @@ static int num_cus_x() {
     return n;
 }
 
-template <typename R>
-static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int axis, const int* mas_p, double* out,
-                   hipStream_t st) {
-    constexpr int NL = 16, T = 256;
+template <typename R, int NL, int T>
+static int xpk_run_k(const void* const* deltas, int F, int N, double BoxSize, int axis, const int* mas_p, double* out,
+                     hipStream_t st) {
     int rc = OK;
     const int64_t P = half_len(N);
     const PkGeometry* g = pk_geometry(N, axis, &rc);
@@ -461,6 +460,17 @@ static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int 
         o += n2;
     }
     return OK;
+}
+
+
+// 16 columns per block (whole 128-byte lines per row read); with the line of sight along z the blocks
+// keep a 2-D histogram row per column, and 8 columns let all pairs of two fields share one launch
+// (16 needed two, each reading both spectra: 5.3 -> 3.7 ms for two 512^3 fields)
+template <typename R>
+static int xpk_run(const void* const* deltas, int F, int N, double BoxSize, int axis, const int* mas_p, double* out,
+                   hipStream_t st) {
+    if (axis == 2) return xpk_run_k<R, 8, 256>(deltas, F, N, BoxSize, axis, mas_p, out, st);
+    return xpk_run_k<R, 16, 256>(deltas, F, N, BoxSize, axis, mas_p, out, st);
 }
 
 }  // namespace pyl
