@@ -242,10 +242,8 @@ def workload_bispectrum(args):
     parts = [synth.zeldovich_device(512, L, seed=200 + r, rms_disp=5.0) for r in range(args.realizations)]
 
     def one(pos):
-        _lib.check(_lib.lib().pyl_mas_deposit(_lib.ptr(pos), None, int(pos.shape[0]), 3, N, L, 2, _lib.ZERO,
-                                              _lib.ptr(grid), _lib.stream()))
-        grid.div_(grid.double().mean().float())
-        grid.sub_(1.0)
+        _lib.check(_lib.lib().pyl_mas_overdensity(_lib.ptr(pos), None, int(pos.shape[0]), N, L, 2, 0,
+                                                  _lib.ptr(grid), _lib.stream()))
         return Pk_library.Bk(grid, L, 0.1, 0.2, theta)
 
     one(parts[0])       # builds the cached indicator sums

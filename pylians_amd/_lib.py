@@ -50,6 +50,7 @@ SIGNATURES = {
     "pyl_dev_zero": [_vp, _i64, _vp],
     "pyl_stream_sync": [_vp],
     "pyl_mas_deposit": [_vp, _vp, _i64, _i, _i, _d, _i, _i, _vp, _vp],
+    "pyl_mas_overdensity": [_vp, _vp, _i64, _i, _d, _i, _i, _vp, _vp],
     "pyl_mas_NGP": [_vp, _vp, _vp, _l, _i, _i, _f, _i],
     "pyl_mas_CIC": [_vp, _vp, _vp, _l, _i, _i, _f, _i],
     "pyl_mas_TSC": [_vp, _vp, _vp, _l, _i, _i, _f, _i],

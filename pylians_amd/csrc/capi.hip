@@ -407,6 +407,32 @@ int pk_particles(const float* pos, const float* W, int64_t n, int dims, double B
 }
 }  // namespace pyl
 
+// The deposit of pyl_pk_particles without the transform: delta = rho / rhobar - 1 on a 3-D grid.
+extern "C" int pyl_mas_overdensity(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
+                                   int flags, void* grid, void* stream) {
+    PYL_CHECK(n > 0 && pos && grid, E_ARG, "bad particle or grid arguments");
+    PYL_CHECK(dims > 0 && BoxSize > 0, E_ARG, "dims and BoxSize must be positive");
+    PYL_CHECK(scheme >= 1 && scheme <= 4, E_ARG, "unknown mass-assignment scheme %d", scheme);
+    hipStream_t st = (hipStream_t)stream;
+    const bool f64 = flags & PYL_GRID_F64;
+    int rc = OK;
+    double* dmass = nullptr;
+    if (W) {
+        dmass = (double*)workspace(WS_MASS, 64, &rc);
+        if (rc) return rc;
+        rc = weight_sum(W, n, dmass, st);
+        if (rc) return rc;
+    }
+    int kind;
+    rc = deposit_kind(pos, W, n, dims, BoxSize, scheme, f64, flags, grid, dmass, st, &kind, Rsd());
+    if (rc) return rc;
+    rc = deposit_delta(pos, W, n, dims, BoxSize, scheme, f64, kind, grid, dmass, st, Rsd(), flags);
+    if (rc) return rc;
+    // the float64 atomic deposit leaves rho in the grid
+    if (f64 && kind == DK_ATOMIC) return delta_inplace((double*)grid, (int64_t)dims * dims * dims, dmass, (double)n, st);
+    return OK;
+}
+
 extern "C" int pyl_pk_particles(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
                                 int axis, int prec, int flags, double* out, void* stream) {
     return pk_particles(pos, W, n, dims, BoxSize, scheme, axis, prec, flags, out, (hipStream_t)stream, Rsd());

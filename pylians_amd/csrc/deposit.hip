@@ -126,6 +126,19 @@ __global__ void k_delta_round(const double* __restrict__ rho, float* __restrict_
         out[i] = (float)(rho[i] * a - 1.0);
 }
 
+// delta = rho * ncell / M - 1 in place on a float64 grid
+__global__ void k_delta_inplace(double* __restrict__ g, long long n, const double* __restrict__ dmass, double mass,
+                                double ncell) {
+    const double a = ncell / (dmass ? dmass[0] : mass);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) g[i] = g[i] * a - 1.0;
+}
+int delta_inplace(double* g, int64_t n, const double* dmass, double mass, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_delta_inplace, dim3(blocks), dim3(256), 0, st, g, (long long)n, dmass, mass, (double)n);
+    PYL_HIP(hipGetLastError());
+    return OK;
+}
 int delta_round(const double* rho, float* out, int64_t n, const double* dmass, double mass, hipStream_t st) {
     const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_delta_round, dim3(blocks), dim3(256), 0, st, rho, out, (long long)n, dmass, mass, (double)n);

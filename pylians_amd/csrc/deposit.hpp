@@ -52,6 +52,8 @@ int deposit(const float* pos, const float* W, int64_t n, int axes, int dims, dou
 int grid_divide(void* grid, int64_t n, bool f64, double s, hipStream_t st);
 // out[i] = (float)(rho[i] * n / mass - 1) in float64 (mass = dmass[0] when dmass is non-null).
 int delta_round(const double* rho, float* out, int64_t n, const double* dmass, double mass, hipStream_t st);
+// delta = rho * n / M - 1 in place on a float64 grid of n cells (M: dmass[0] when given, else mass)
+int delta_inplace(double* g, int64_t n, const double* dmass, double mass, hipStream_t st);
 
 // Cell-binned LDS deposit (3-D grids with dims % 16 == 0).  mode: 0 = grid += rho, 1 = grid = rho,
 // 2 = grid = rho * N^3 / mass - 1 with mass = dmass[0] when dmass != nullptr, else `mass`.

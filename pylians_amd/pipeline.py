@@ -8,6 +8,9 @@ It exposes the same attributes as ``Pk_library.Pk``.
 With ``vel``, the redshift-space shift of pos_redshift_space (redshift_space_library.pyx:29-43)
 is applied to each position as the deposit loads it.
 
+``overdensity(pos, BoxSize, dims, MAS='CIC', W=None, precision='fp32')`` stops after the deposit and
+returns delta on the device (the input of Pk_library.Bk / XPk / Xi).
+
 ``run_device`` is the allocation-free inner call used by the benchmark: device inputs, a
 device output vector; with ``flags | PK_ASYNC`` no host synchronisation (the caller then checks
 the returned Nmodes, which the binning kernel counted).
@@ -67,3 +70,22 @@ class PkParticles:
                 _modes_exit(e.code)
             raise
         unpack_pk(self, _lib.to_host(out), dims)
+
+
+def overdensity(pos, BoxSize, dims, MAS="CIC", W=None, precision="fp32", deposit=None):
+    """delta = rho / rhobar - 1 on a dims^3 device grid (float32, or float64 with precision='fp64'),
+    the MA call plus normalisation of Pk_snapshot.py:84-88 in one entry point: the deposit's float64
+    cell sums are normalised before the single rounding."""
+    if MAS not in _lib.SCHEMES:
+        raise ValueError("unknown MAS %r" % (MAS,))
+    p = _lib.to_dev(pos, _lib.F32T)
+    if p.ndim != 2 or p.shape[1] != 3:
+        raise ValueError("pos must be (n, 3)")
+    w = None if W is None else _lib.to_dev(W, _lib.F32T)
+    f64 = PRECISION[precision] == _lib.F64
+    grid = _lib.empty((dims,) * 3, _lib.F64T if f64 else _lib.F32T)
+    f = {None: 0, "atomic": _lib.ATOMIC, "binned": _lib.BINNED, "presorted": _lib.SORTED}[deposit]
+    _lib.check(_lib.lib().pyl_mas_overdensity(_lib.ptr(p), _lib.ptr(w), int(p.shape[0]), int(dims), float(BoxSize),
+                                              _lib.SCHEMES[MAS], f | (_lib.GRID_F64 if f64 else 0), _lib.ptr(grid),
+                                              _lib.stream()))
+    return grid

@@ -280,3 +280,62 @@ def test_dense_cic_register_runs(gpu, ordered, weighted):
     assert np.max(np.abs(gacc - (start + o)) / (start + o)) <= 6.5e-8
     g64 = _dev_deposit(pos, W, dims, L, 2, _lib.BINNED | _lib.ZERO, f64=True)
     assert np.allclose(g64, o, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_overdensity_entry_point(gpu, scheme, weighted):
+    """pyl_mas_overdensity: delta = rho * dims^3 / M - 1 (Pk_snapshot.py:84-88) from every deposit path
+    (atomic, binned, presorted, the autotuner's choice), into float32 and float64 grids.  float32: one
+    rounding of the float64 value; float64: the oracle's value to 1e-12."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(47)
+    dims, L = 48, 150.0
+    pos = synth.zeldovich(64, L, seed=6, rms_disp=3.0, sigma_v=80.0)
+    W = (rng.random(len(pos), dtype=np.float32) + np.float32(0.1)) if weighted else None
+    rho = C.deposit_f64(pos, dims, L, scheme, W)
+    mass = float(len(pos)) if W is None else float(np.sum(W, dtype=np.float64))
+    d = rho * (dims ** 3 / mass) - 1.0
+    p = torch.from_numpy(pos).cuda()
+    w = None if W is None else torch.from_numpy(W).cuda()
+    s = _lib.SCHEMES[scheme]
+    for flags in (_lib.ATOMIC, _lib.BINNED, _lib.SORTED, 0):
+        for f64 in (False, True):
+            g = torch.full((dims,) * 3, 7.0, dtype=torch.float64 if f64 else torch.float32, device="cuda")
+            _lib.check(_lib.lib().pyl_mas_overdensity(_lib.ptr(p), _lib.ptr(w), len(pos), dims, L, s,
+                                                      flags | (_lib.GRID_F64 if f64 else 0), _lib.ptr(g),
+                                                      _lib.stream()))
+            h = g.cpu().numpy()
+            if f64:
+                assert np.allclose(h, d, rtol=1e-12, atol=1e-12), flags
+            else:
+                err = np.abs(h.astype(np.float64) - d)
+                assert np.all(err <= 6.5e-8 * np.abs(d) + 1e-12), (flags, err.max())
+
+
+def test_overdensity_errors(gpu):
+    from pylians_amd import _lib
+    g = torch.zeros((40,) * 3, device="cuda")
+    p = torch.rand((100, 3), device="cuda") * 10
+    L = _lib.lib()
+    assert L.pyl_mas_overdensity(_lib.ptr(p), None, 100, 40, 10.0, 9, 0, _lib.ptr(g), _lib.stream()) == _lib.E_ARG
+    assert L.pyl_mas_overdensity(_lib.ptr(p), None, 0, 40, 10.0, 2, 0, _lib.ptr(g), _lib.stream()) == _lib.E_ARG
+    # dims 40 is not a multiple of 16: the binned deposit is refused, the atomic one runs
+    assert L.pyl_mas_overdensity(_lib.ptr(p), None, 100, 40, 10.0, 2, _lib.BINNED, _lib.ptr(g),
+                                 _lib.stream()) == _lib.E_UNSUPPORTED
+    _lib.check(L.pyl_mas_overdensity(_lib.ptr(p), None, 100, 40, 10.0, 2, 0, _lib.ptr(g), _lib.stream()))
+    assert abs(float(g.double().mean())) < 1e-6
+
+
+def test_overdensity_python_wrapper(gpu):
+    """pipeline.overdensity from host arrays equals the C entry point's default path."""
+    from pylians_amd import pipeline
+    pos = synth.zeldovich(32, 80.0, seed=3, rms_disp=2.0, sigma_v=50.0)
+    rho = C.deposit_f64(pos, 32, 80.0, "TSC", None)
+    d = rho * (32 ** 3 / len(pos)) - 1.0
+    g = pipeline.overdensity(pos, 80.0, 32, "TSC")
+    h = np.asarray(g.cpu().numpy() if hasattr(g, "cpu") else g.numpy(), np.float64)
+    assert np.all(np.abs(h - d) <= 6.5e-8 * np.abs(d) + 1e-12)
+    g64 = pipeline.overdensity(pos, 80.0, 32, "TSC", precision="fp64")
+    h64 = g64.cpu().numpy() if hasattr(g64, "cpu") else g64.numpy()
+    assert np.allclose(h64, d, rtol=1e-12, atol=1e-12)
