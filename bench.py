@@ -367,6 +367,16 @@ def main():
                           "stage_ms": {k: round(v[1] / v[0], 4) for k, v in stages.items() if v[0] > 0}}), flush=True)
         return
 
+    # per-step times (a second pass of max(K, 10) steps, one event pair each): the median step
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(max(args.steps, 10))]
+    for a, b in evs:
+        a.record()
+        step(_lib.F32)
+        b.record()
+    torch.cuda.synchronize()
+    ms_med = float(np.median([a.elapsed_time(b) for a, b in evs]))
+
     # ---- plain float64 GPU path (same particles, float64 grid, FFT and sums)
     step(_lib.F64)
     torch.cuda.synchronize()
@@ -411,7 +421,8 @@ def main():
 
     line = {
         "metric": METRIC, "value": round(n_p / (ms * 1e-3) / 1e6, 1), "unit": "Mparticles/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "median_ms_per_step": round(ms_med, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (Zel'dovich lattice + RSD, seeded, generated on the GPU)",
         "config": {"workload": workload, "n_particles": n_p,

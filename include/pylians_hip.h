@@ -93,9 +93,6 @@ int pyl_stream_sync(void* stream);
 int pyl_mas_deposit(const float* pos, const float* W, int64_t n, int axes, int dims, double BoxSize,
                     int scheme, int flags, void* grid, void* stream);
 
-/* Host-pointer drop-ins of the OpenMP C kernels, same argument list as MAS_c.h:3-10
- * (void NGP|CIC|TSC|PCS(FLOAT *pos, FLOAT *number, FLOAT *W, long particles, int dims, int axes,
- * FLOAT BoxSize, int threads)).  Synchronous; `threads` is accepted and ignored. */
 /* Overdensity of particles on a 3-D grid, delta = rho / rhobar - 1 with rhobar = M / dims^3 (M = n, or
  * the float64 sum of W): the deposit of pyl_pk_particles, written to `grid` (float32, float64 with
  * PYL_GRID_F64; overwritten), delta evaluated in float64 before one rounding.  Replaces MA followed by
@@ -104,6 +101,10 @@ int pyl_mas_deposit(const float* pos, const float* W, int64_t n, int axes, int d
  * is the autotuner's choice. */
 int pyl_mas_overdensity(const float* pos, const float* W, int64_t n, int dims, double BoxSize, int scheme,
                         int flags, void* grid, void* stream);
+
+/* Host-pointer drop-ins of the OpenMP C kernels, same argument list as MAS_c.h:3-10
+ * (void NGP|CIC|TSC|PCS(FLOAT *pos, FLOAT *number, FLOAT *W, long particles, int dims, int axes,
+ * FLOAT BoxSize, int threads)).  Synchronous; `threads` is accepted and ignored. */
 int pyl_mas_NGP(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,
                 int threads);
 int pyl_mas_CIC(float* pos, float* number, float* W, long particles, int dims, int axes, float BoxSize,

@@ -308,6 +308,7 @@ extern "C" int pyl_bk(const void* delta, int dims, double BoxSize, double k1, do
     PYL_CHECK(prec == PYL_F32 || prec == PYL_F64, E_ARG, "prec must be 4 or 8");
     PYL_CHECK(fft_size_supported(dims), E_UNSUPPORTED, "grid size %d not supported", dims);
     PYL_CHECK(ntheta >= 0 && (ntheta == 0 || theta), E_ARG, "bad theta array");
+    PYL_CHECK(BoxSize > 0, E_ARG, "BoxSize must be positive");
     PYL_CHECK(delta && B && Q && k && Pk, E_ARG, "null pointer");
     PYL_CHECK(mas_p >= 0 && mas_p <= 4, E_ARG, "window exponent must be in [0, 4]");
     hipStream_t st = (hipStream_t)stream;
