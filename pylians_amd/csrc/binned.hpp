@@ -23,6 +23,7 @@ struct BinArgs {
     long long* ov_idx;   // [n]
     unsigned* ov_key;    // [n]
     Rsd rsd;             // redshift-space shift of the positions (rsd.vel == nullptr: none)
+    int agg = 0;         // runs of equal keys in consecutive lanes counted once (bin_agg)
 };
 
 struct TileArgs {
@@ -109,5 +110,7 @@ int binned_batch_launch(int scheme, bool hasw, bool f64, int acc, bool sorted, c
 // weights with register runs of the stencil block) for CIC inputs with at least RUN_PPC
 // particles per cell
 int dense_acc(int acc, int scheme, int64_t n, int dims);
+// 1 when k_bin counts runs of equal keys in consecutive lanes once (BinArgs::agg): n >= 2 dims^3
+int bin_agg(int64_t n, int dims);
 
 }  // namespace pyl

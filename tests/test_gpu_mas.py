@@ -339,3 +339,35 @@ def test_overdensity_python_wrapper(gpu):
     g64 = pipeline.overdensity(pos, 80.0, 32, "TSC", precision="fp64")
     h64 = g64.cpu().numpy() if hasattr(g64, "cpu") else g64.numpy()
     assert np.allclose(h64, d, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("weighted", [False, True])
+def test_binned_run_counting(gpu, scheme, weighted):
+    """From 2 particles per cell the bucketing pass counts runs of equal (tile, class) keys in
+    consecutive lanes once (lattice-ordered particles: runs of tens of lanes).  Ordered particles at 8
+    per cell with a particle count that leaves the last wave ragged (lanes past the end form runs of
+    their own), plain and presorted, deposit and overdensity: every cell the single rounding of its
+    float64 sum, as without the run counting (MAS_library.pyx:123-545)."""
+    from pylians_amd import _lib
+    rng = np.random.default_rng(53)
+    dims, L = 32, 90.0
+    pos = synth.zeldovich(64, L, seed=14, rms_disp=1.5, sigma_v=40.0)[:-777]
+    assert len(pos) >= 2 * dims ** 3 and len(pos) % 2048 != 0
+    W = (rng.random(len(pos), dtype=np.float32) + np.float32(0.1)) if weighted else None
+    o = C.deposit_f64(pos, dims, L, scheme, W)
+    s = _lib.SCHEMES[scheme]
+    for flags in (_lib.BINNED, _lib.SORTED):
+        g = _dev_deposit(pos, W, dims, L, s, flags | _lib.ZERO)
+        assert np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-30)) <= 6.5e-8, flags
+        g64 = _dev_deposit(pos, W, dims, L, s, flags | _lib.ZERO, f64=True)
+        assert np.allclose(g64, o, rtol=1e-13, atol=1e-13), flags
+    mass = float(len(pos)) if W is None else float(np.sum(W, dtype=np.float64))
+    d = o * (dims ** 3 / mass) - 1.0
+    p = torch.from_numpy(pos).cuda()
+    w = None if W is None else torch.from_numpy(W).cuda()
+    g = torch.empty((dims,) * 3, device="cuda")
+    _lib.check(_lib.lib().pyl_mas_overdensity(_lib.ptr(p), _lib.ptr(w), len(pos), dims, L, s, _lib.BINNED,
+                                              _lib.ptr(g), _lib.stream()))
+    h = g.cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(h - d) <= 6.5e-8 * np.abs(d) + 1e-12)
