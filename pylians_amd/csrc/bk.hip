@@ -192,10 +192,22 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
     R* f = (R*)workspace(WS_AUX, sizeof(R) * ncell * 4, &rc);
     if (rc) return rc;
     R *d1 = f, *d12 = f + ncell, *I1 = f + 2 * ncell, *I12 = f + 3 * ncell;
-    const size_t nacc = 4 * (size_t)(nb + 2);
-    double* acc = (double*)workspace(WS_SUMS, sizeof(double) * nacc, &rc);
+    // theta shells: the mode sums are binned by the integer |k|^2 = m in [mlo, mhi)
+    double lo = 1e300, hi = 0.0;
+    for (int j = 0; j < nb; ++j) {
+        lo = std::min(lo, kmin[j + 2]);
+        hi = std::max(hi, kmax[j + 2]);
+    }
+    const int mlo = nb ? (int)std::floor(std::max(lo, 0.0)) * (int)std::floor(std::max(lo, 0.0)) : 0;
+    const int mhi = nb ? std::max(mlo, (int)std::ceil(std::max(hi, 0.0)) * (int)std::ceil(std::max(hi, 0.0)) + 1) : 0;
+    const size_t nbin = (size_t)(mhi - mlo);
+    // the shell sums (4 per shell) and the 4 sums per m bin share one buffer: one clear, one copy back
+    const size_t nacc = 4 * (size_t)(nb + 2), nres = nacc + 4 * nbin;
+    double* acc = (double*)workspace(WS_SUMS, sizeof(double) * nres, &rc);
     if (rc) return rc;
-    rc = zero_fill(acc, sizeof(double) * nacc, st);
+    rc = zero_fill(acc, sizeof(double) * nres, st);
+    if (rc) return rc;
+    double* hres = (double*)host_pinned(sizeof(double) * nres, &rc);
     if (rc) return rc;
     int dev = 0;
     PYL_HIP(hipGetDevice(&dev));
@@ -218,15 +230,6 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
         if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[1], kmax[1], 1, win, I12, 2, I1, acc + 6, st))) return rc;
     }
     // theta shells: P = FFT(d1 d2) (and FFT(I1 I2)), then the mode sums binned by |k|^2
-    double lo = 1e300, hi = 0.0;
-    for (int j = 0; j < nb; ++j) {
-        lo = std::min(lo, kmin[j + 2]);
-        hi = std::max(hi, kmax[j + 2]);
-    }
-    const int mlo = nb ? (int)std::floor(std::max(lo, 0.0)) * (int)std::floor(std::max(lo, 0.0)) : 0;
-    const int mhi = nb ? std::max(mlo, (int)std::ceil(std::max(hi, 0.0)) * (int)std::ceil(std::max(hi, 0.0)) + 1) : 0;
-    const size_t nbin = (size_t)(mhi - mlo);
-    std::vector<double> hb(4 * nbin);
     if (nbin) {
         cplx<R>* pd = ws;   // the shell transforms are done with it
         cplx<R>* pi = nullptr;
@@ -236,28 +239,37 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
             if (rc) return rc;
             if ((rc = fft3d_r2c<R>(I12, pi, N, P, st))) return rc;
         }
-        double* bins = (double*)workspace(WS_BINS, sizeof(double) * 4 * nbin, &rc);
-        if (rc) return rc;
-        if ((rc = zero_fill(bins, sizeof(double) * 4 * nbin, st))) return rc;
+        double* bins = acc + nacc;
         const long long total = (long long)N * N * K;
         const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 8192);
         hipLaunchKernelGGL(k_bk_modes<R>, dim3(blocks), dim3(256), 0, st, spec, pd, pi, (long long)P, N, K, mlo, mhi,
                            win, bins);
         PYL_HIP(hipGetLastError());
-        PYL_HIP(hipMemcpyAsync(hb.data(), bins, sizeof(double) * 4 * nbin, hipMemcpyDeviceToHost, st));
     }
-    std::vector<double> h(nacc);
-    PYL_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st));
+    PYL_HIP(hipMemcpyAsync(hres, acc, sizeof(double) * nres, hipMemcpyDeviceToHost, st));
     PYL_HIP(hipStreamSynchronize(st));
-    // shell j: the bins kmin <= sqrt(m) < kmax; by Parseval each sum carries 1/N^3
+    std::vector<double> h(hres, hres + nacc);
+    const double* hb = hres + nacc;
+    // first m in [mlo, mhi] with sqrt(m) >= kb: sqrt is monotone, so the bins of a shell
+    // (kmin <= sqrt(m) < kmax) are the contiguous range [first(kmin), first(kmax))
+    auto first_ge = [&](double kb) -> int {
+        if (kb != kb) return mhi;
+        if (!(kb > 0.0)) return mlo;
+        const double m0 = std::floor(kb * kb) - 2.0;
+        long long m = m0 < (double)mlo ? mlo : (m0 > (double)mhi ? mhi : (long long)m0);
+        while (m > mlo && sqrt((double)(m - 1)) >= kb) --m;
+        while (m < mhi && sqrt((double)m) < kb) ++m;
+        return (int)m;
+    };
+    // shell j: the bins kmin <= sqrt(m) < kmax, summed in increasing m; by Parseval each sum
+    // carries 1/N^3
     const double inv3 = 1.0 / ((double)N * N * N);
     for (int j = 0; j < nb; ++j) {
         double* a = h.data() + 4 * (j + 2);
         double sdp = 0.0, sdd = 0.0, sip = 0.0, sii = 0.0;
-        for (int m = mlo; m < mhi; ++m) {
-            const double k = sqrt((double)m);
-            if (!(k >= kmin[j + 2] && k < kmax[j + 2])) continue;
-            const double* b = hb.data() + 4 * (size_t)(m - mlo);
+        const int ma = first_ge(kmin[j + 2]), mb = first_ge(kmax[j + 2]);
+        for (int m = ma; m < mb; ++m) {
+            const double* b = hb + 4 * (size_t)(m - mlo);
             sdp += b[0];
             sdd += b[1];
             sip += b[2];

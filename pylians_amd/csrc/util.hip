@@ -153,6 +153,36 @@ void* workspace(int slot, size_t bytes, int* rc) {
     return grow(e, bytes, rc);
 }
 
+void* host_pinned(size_t bytes, int* rc) {
+    static std::mutex mu;
+    static std::map<int, std::pair<void*, size_t>> bufs;
+    *rc = OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *rc = E_LAUNCH;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto& e = bufs[dev];
+    if (e.first && e.second >= bytes) return e.first;
+    if (e.first) {
+        (void)hipDeviceSynchronize();
+        (void)hipHostFree(e.first);
+        e = std::make_pair(nullptr, (size_t)0);
+    }
+    const size_t want = bytes < 4096 ? 4096 : bytes;
+    if (hipHostMalloc(&e.first, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        e.first = nullptr;
+        set_error("pinned host allocation of %zu bytes failed", want);
+        *rc = E_ALLOC;
+        return nullptr;
+    }
+    e.second = want;
+    return e.first;
+}
+
 WsArena* ws_arena_create() {
     WsArena* a = new WsArena();
     if (hipGetDevice(&a->dev) != hipSuccess) a->dev = 0;

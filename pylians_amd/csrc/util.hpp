@@ -22,6 +22,10 @@ enum WsSlot { WS_SPEC = 0, WS_SPEC2 = 1, WS_SUMS = 2, WS_GRID = 3, WS_BINS = 4, 
               WS_PART = 13, WS_STAGE = 14, WS_SUP = 15, WS_C2R = 16,
               WS_RED = 17, WS_BATCH = 18, WS_BATCH_TABLES = 19 };
 void* workspace(int slot, size_t bytes, int* rc);
+// Library-owned grow-only pinned host buffer, one per device, for the results a call copies back
+// before it returns (one asynchronous copy, then a stream synchronisation).  Growing synchronises the
+// device before freeing the old buffer.
+void* host_pinned(size_t bytes, int* rc);
 
 // A private set of workspaces (one buffer per slot) owned by a captured hipGraph.  While an
 // ArenaScope is alive on a thread, workspace() on that thread serves the arena's buffers instead of
