@@ -207,8 +207,8 @@ int pyl_pos_redshift_space(float* pos, const float* vel, int64_t n, int cols, fl
  * (1..8) fields.  deltas: HOST array of nfields device pointers to (N,N,N) reals of precision prec;
  * mas_p: HOST array of window exponents.  out: HOST float64 array of pyl_xpk_layout values:
  *   k3D[kmax] Nmodes3D[kmax] k1D[kmax_par] Nmodes1D[kmax_par] kpar[n2] kper[n2] Nmodes2D[n2]
- *   then per pair (autos 0..F-1, then crosses (0,1),(0,2),...,(1,2),...):
- *   P[kmax][3] (P0,P2,P4)  P1D[kmax_par]  P2D[n2].
+ *   then, over the C = F(F+1)/2 pairs (autos 0..F-1, then crosses (0,1),(0,2),...,(1,2),...) as the
+ *   last, fastest index: P[kmax][3][C] (P0,P2,P4)  P1D[kmax_par][C]  P2D[n2][C].
  * The call synchronises `stream`. */
 int pyl_xpk_layout(int dims, int nfields, int64_t* out_len);
 int pyl_xpk(const void* const* deltas, int nfields, int dims, double BoxSize, int axis, const int* mas_p, int prec,

@@ -226,7 +226,7 @@ class XPk:
         mas = (ctypes.c_int * fields)(*[MAS_function(m) for m in MAS])
         n = ctypes.c_int64()
         _lib.check(_lib.lib().pyl_xpk_layout(dims, fields, ctypes.byref(n)))
-        out = np.zeros(n.value)
+        out = np.empty(n.value)   # pyl_xpk writes every value of the layout
         rc = _lib.lib().pyl_xpk(ptrs, fields, dims, float(BoxSize), int(axis), mas, _lib.F64 if f64 else _lib.F32,
                                 out.ctypes.data_as(ctypes.c_void_p), _lib.stream())
         if rc == _lib.E_MODES:
@@ -246,14 +246,10 @@ class XPk:
         self.k1D, self.Nmodes1D = take(kmax_par), take(kmax_par)
         self.kpar, self.kper, self.Nmodes2D = take(n2), take(n2), take(n2)
         C = fields * (fields + 1) // 2
-        P3, P1, P2 = [], [], []
-        for _ in range(C):
-            P3.append(take(3 * kmax).reshape(kmax, 3))
-            P1.append(take(kmax_par))
-            P2.append(take(n2))
-        P3 = np.stack(P3, axis=2)
-        P1 = np.stack(P1, axis=1)
-        P2 = np.stack(P2, axis=1)
+        # the pairs are the last index of each block (autos, then crosses): views, no copies
+        P3 = take(3 * kmax * C).reshape(kmax, 3, C)
+        P1 = take(kmax_par * C).reshape(kmax_par, C)
+        P2 = take(n2 * C).reshape(n2, C)
         self.Pk, self.XPk = P3[:, :, :fields], P3[:, :, fields:]
         self.Pk1D, self.PkX1D = P1[:, :fields], P1[:, fields:]
         self.Pk2D, self.PkX2D = P2[:, :fields], P2[:, fields:]

@@ -10,6 +10,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/pylians_hip.h"
@@ -44,6 +46,20 @@ struct XBinArgs {
 };
 
 // NF: register slots for the distinct fields of one launch (>= their number); NC: its pairs
+// 1 / (mode count) of every 2-D bin of a geometry (inf for an empty bin, whose sum is 0: NaN, as
+// 0/0), cached per (grid size, line of sight), which fix the counts
+static const std::vector<double>& inv_counts(const PkGeometry* g) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& v = cache[std::make_pair(g->d.N, g->axis)];
+    if (v.size() != g->h_n2.size()) {
+        v.resize(g->h_n2.size());
+        for (size_t i = 0; i < v.size(); ++i) v[i] = 1.0 / (double)g->h_n2[i];
+    }
+    return v;
+}
+
 template <typename R, int NL, int T, int NF, int NC>
 __global__ __launch_bounds__(T) void k_xbin(XBinArgs a) {
     constexpr int TPC = T / NL;
@@ -393,12 +409,15 @@ static int xpk_run_k(const void* const* deltas, int F, int N, double BoxSize, in
         rc = launch_xbin<R, NL, T>(a, grid, shm, st);
         if (rc) return rc;
     }
-    std::vector<double> h(nsums);
-    PYL_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    // one copy into the library's pinned buffer (no pageable staging, no per-call host allocation)
+    double* hp = (double*)host_pinned(sizeof(double) * nsums, &rc);
+    if (rc) return rc;
+    PYL_HIP(hipMemcpyAsync(hp, sums, sizeof(double) * nsums, hipMemcpyDeviceToHost, st));
     PYL_HIP(hipStreamSynchronize(st));
+    const double* h = hp;
     // Nmodes from the kernel's counts, checked bin by bin against the independent-mode geometry
     // (check_number_modes, Pk_library.pyx:90-102, 785)
-    const double* k3c = h.data() + (size_t)per * C;
+    const double* k3c = h + (size_t)per * C;
     const double* k1c = k3c + d.n3();
     const double* k2c = k1c + d.n1();
     {
@@ -430,34 +449,42 @@ static int xpk_run_k(const void* const* deltas, int F, int N, double BoxSize, in
     o += kp;
     for (int i = 0; i < kp; ++i) o[i] = k1c[i + 1];
     o += kp;
-    for (int i = 0; i < n2; ++i) o[i] = 0.5 * (2 * (i % (kp + 1)) + 1) * kF;
-    o += n2;
-    for (int i = 0; i < n2; ++i) o[i] = 0.5 * (2 * (i / (kp + 1)) + 1) * kF;
-    o += n2;
+    // 2-D bin i = (kmax_par + 1) * kper + kpar
+    for (int i = 0, r = 0; i < n2; ++r)
+        for (int q = 0; q <= kp; ++q, ++i) {
+            o[i] = 0.5 * (2 * q + 1) * kF;
+            o[n2 + i] = 0.5 * (2 * r + 1) * kF;
+        }
+    o += 2 * (size_t)n2;
     for (int i = 0; i < n2; ++i) o[i] = k2c[i];
     o += n2;
+    // 1/Nmodes2D of the geometry (equal to the kernel's counts, checked above), once per geometry:
+    // one division per 2-D bin instead of one per bin and pair
+    const std::vector<double>& inv2 = inv_counts(g);
+    // the pairs interleaved, as the reference's arrays Pk[kmax][3][F] / Pk1D[kmax_par][F] / Pk2D[n2][F]
+    // hold them: P[kmax][3][C], then P1D[kmax_par][C], then P2D[n2][C]
+    double* o3 = o;
+    double* o1 = o3 + (size_t)3 * kmax * C;
+    double* o2 = o1 + (size_t)kp * C;
     for (int c = 0; c < C; ++c) {
         // locate this pair's sums inside its launch group
         const int c0 = (c / cmax) * cmax, nc = (C - c0 < cmax ? C - c0 : cmax), cc = c - c0;
-        const double* gb = h.data() + (size_t)per * c0;
+        const double* gb = h + (size_t)per * c0;
         const double* s3 = gb + ((size_t)cc * d.n3()) * 3;
         const double* s1 = gb + (size_t)3 * d.n3() * nc + (size_t)cc * d.n1();
         const double* s2 = gb + (size_t)3 * d.n3() * nc + (size_t)d.n1() * nc + (size_t)cc * n2;
         for (int i = 0; i < kmax; ++i) {
             const double n = k3c[i + 1];
-            o[3 * i + 0] = (s3[3 * (i + 1) + 0] / n) * fact;
-            o[3 * i + 1] = (s3[3 * (i + 1) + 1] * 5.0 / n) * fact;
-            o[3 * i + 2] = (s3[3 * (i + 1) + 2] * 9.0 / n) * fact;
+            o3[(size_t)(3 * i + 0) * C + c] = (s3[3 * (i + 1) + 0] / n) * fact;
+            o3[(size_t)(3 * i + 1) * C + c] = (s3[3 * (i + 1) + 1] * 5.0 / n) * fact;
+            o3[(size_t)(3 * i + 2) * C + c] = (s3[3 * (i + 1) + 2] * 9.0 / n) * fact;
         }
-        o += 3 * kmax;
         for (int i = 0; i < kp; ++i) {
             const double n = k1c[i + 1];
             const double kmaxper = sqrt(kN * kN - k1D[i] * k1D[i]);
-            o[i] = s1[i + 1] * fact * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
+            o1[(size_t)i * C + c] = s1[i + 1] * fact * (M_PI * kmaxper * kmaxper / n) / ((2.0 * M_PI) * (2.0 * M_PI));
         }
-        o += kp;
-        for (int i = 0; i < n2; ++i) o[i] = s2[i] * fact / k2c[i];
-        o += n2;
+        for (int i = 0; i < n2; ++i) o2[(size_t)i * C + c] = s2[i] * fact * inv2[i];
     }
     return OK;
 }
