@@ -269,14 +269,27 @@ constexpr int bin_rows(int N, int T, int NL) {
     const int rows = eu % 2 == 0 ? N + 2 * ((N / 2) / eu) + 2 : N;
     return rows > N + T / NL ? rows : N + T / NL;
 }
-// lines per block: at most 8, and few enough that two blocks share a CU (160 KB of LDS) beside their
-// histograms
-template <typename R, int N, bool FIXK> constexpr int nl_bin() {
+// LDS bytes of one binning block: the mapped rows, the twiddle and window tables, the histograms
+template <typename R> constexpr int lds_bin(int N, int T, int NL, bool fixk) {
+    return bin_rows(N, T, NL) * (NL + 1) * (int)sizeof(cplx<R>) + N * (int)(sizeof(cplx<R>) + sizeof(R)) +
+           hist_bytes(N, NL, fixk);
+}
+// lines per block when two blocks share a CU (160 KB of LDS) beside their histograms: at most 8
+template <typename R, int N, bool FIXK> constexpr int nl_pair() {
     int nl = 8;
-    while (nl > 2 && bin_rows(N, threads_bin_c(N, nl, FIXK), nl) * (nl + 1) * (int)sizeof(cplx<R>) +
-                             N * (int)(sizeof(cplx<R>) + sizeof(R)) + hist_bytes(N, nl, FIXK) > 80 * 1024)
-        nl /= 2;
+    while (nl > 2 && lds_bin<R>(N, threads_bin_c(N, nl, FIXK), nl, FIXK) > 80 * 1024) nl /= 2;
     return nl;
+}
+// Where two blocks per CU leave only 2 lines per block (float64 lines of 512 points, float32 from 768
+// along x or y: 32- or 16-byte column segments and 1-2 waves per block), one block per CU with 8 lines
+// and N threads (at most 512) when that fits (float64 at 512, line of sight x: 1.42 -> 0.69 ms)
+constexpr int SOLO_LDS = 150 * 1024;
+template <typename R, int N, bool FIXK> constexpr bool solo_bin() {
+    return nl_pair<R, N, FIXK>() == 2 && lds_bin<R>(N, clamp_i(N, 64, 512), 8, FIXK) <= SOLO_LDS;
+}
+template <typename R, int N, bool FIXK> constexpr int nl_bin() { return solo_bin<R, N, FIXK>() ? 8 : nl_pair<R, N, FIXK>(); }
+template <typename R, int N, bool FIXK> constexpr int threads_bin() {
+    return solo_bin<R, N, FIXK>() ? clamp_i(N, 64, 512) : threads_bin_c(N, nl_bin<R, N, FIXK>(), FIXK);
 }
 
 
@@ -734,7 +747,7 @@ int pk_last_axis(int axis) { return axis == 0 ? 1 : 0; }
 // column blocks with prefetch (four per CU with fixk measured faster)
 template <typename R, int N, bool FIXK> struct BinShape {
     static constexpr int NL = nl_bin<R, N, FIXK>();
-    static constexpr int T = threads_bin_c(N, NL, FIXK);
+    static constexpr int T = threads_bin<R, N, FIXK>();
     static constexpr int W = N / 2 + 1;   // kz = 0 .. N/2; rows of the spectrum may be padded to P >= W
     int nchunk, grid, n2w, RS, n31;
     long long n2;

@@ -71,6 +71,11 @@ def test_zeldovich512_rsd_grid512(zeldovich512, MAS):
         # outright in every bin
         assert e0.max() <= TOL and e2.max() <= TOL and e4.max() <= TOL
         assert res["phase"] <= TOL and res["Pk1D"] <= TOL
+        # the float64 path (float64 grid, complex128 transform; its binning blocks take one CU each
+        # with 8 lines of 512 points) against the same oracle
+        g64 = pipeline.PkParticles(pos, L, N, MAS=MAS, axis=0, precision="fp64")
+        r64 = assert_pk_close(g64, o, 1e-10)
+        print("float64 path 512^3: max rel err P0 %.2e  phase %.2e  Pk2D %.2e" % (r64["P0"], r64["phase"], r64["Pk2D"]))
 
 
 def test_batch64_uneven_graph(gpu):
