@@ -280,16 +280,24 @@ template <typename R, int N, bool FIXK> constexpr int nl_pair() {
     while (nl > 2 && lds_bin<R>(N, threads_bin_c(N, nl, FIXK), nl, FIXK) > 80 * 1024) nl /= 2;
     return nl;
 }
-// Where two blocks per CU leave only 2 lines per block (float64 lines of 512 points, float32 from 768
-// along x or y: 32- or 16-byte column segments and 1-2 waves per block), one block per CU with 8 lines
-// and N threads (at most 512) when that fits (float64 at 512, line of sight x: 1.42 -> 0.69 ms)
+// Where two blocks per CU leave only 2 lines per block (float64 lines from 512 points, float32 from
+// 640: 32- or 16-byte column segments and 1-2 waves per block), one block per CU with 8 lines, or 4
+// where 8 do not fit, and N * lines / 8 threads (at most 512) (float64 at 512: line of sight x
+// 1.42 -> 0.69 ms with 8 lines, z 1.27 -> 0.99 ms with 4)
 constexpr int SOLO_LDS = 150 * 1024;
-template <typename R, int N, bool FIXK> constexpr bool solo_bin() {
-    return nl_pair<R, N, FIXK>() == 2 && lds_bin<R>(N, clamp_i(N, 64, 512), 8, FIXK) <= SOLO_LDS;
+constexpr int solo_threads(int N, int nl) { return clamp_i(N * nl / 8, 64, 512); }
+template <typename R, int N, bool FIXK> constexpr int solo_nl() {
+    if (nl_pair<R, N, FIXK>() != 2) return 0;
+    if (lds_bin<R>(N, solo_threads(N, 8), 8, FIXK) <= SOLO_LDS) return 8;
+    if (lds_bin<R>(N, solo_threads(N, 4), 4, FIXK) <= SOLO_LDS) return 4;
+    return 0;
 }
-template <typename R, int N, bool FIXK> constexpr int nl_bin() { return solo_bin<R, N, FIXK>() ? 8 : nl_pair<R, N, FIXK>(); }
+template <typename R, int N, bool FIXK> constexpr int nl_bin() {
+    return solo_nl<R, N, FIXK>() ? solo_nl<R, N, FIXK>() : nl_pair<R, N, FIXK>();
+}
 template <typename R, int N, bool FIXK> constexpr int threads_bin() {
-    return solo_bin<R, N, FIXK>() ? clamp_i(N, 64, 512) : threads_bin_c(N, nl_bin<R, N, FIXK>(), FIXK);
+    return solo_nl<R, N, FIXK>() ? solo_threads(N, solo_nl<R, N, FIXK>())
+                                 : threads_bin_c(N, nl_bin<R, N, FIXK>(), FIXK);
 }
 
 

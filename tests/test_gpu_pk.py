@@ -154,21 +154,24 @@ def test_xpk_five_fields(PKL):
             assert ok(np.abs(g.XPk[:, l, x] - o.XPk[:, l, x]) / scale, nz.XPk[:, l, x] / scale), ("cross", x, l)
 
 
-@pytest.mark.parametrize("N", [768, 1024])
-def test_pk_large_grids_float32_vs_float64(PKL, N):
-    """Grids of 768 and 1024 points, line of sight along x (Pk_library.pyx:266-425): the float32 binning
-    runs one block per CU with 8 lines (two blocks per CU would hold 2 lines each), the float64 binning
-    the two-per-CU shape.  On white noise without a window the mode counts agree exactly and every
+@pytest.mark.parametrize("axis", [0, 2])
+@pytest.mark.parametrize("N", [512, 640, 768, 1024])
+def test_pk_large_grids_float32_vs_float64(PKL, N, axis):
+    """Grids of 512 to 1024 points, line of sight along x and along z (Pk_library.pyx:266-425): where two
+    binning blocks per CU would hold 2 lines each (float64 from 512 points, float32 from 640), the
+    binning runs one block per CU with 8 or 4 lines; float32 and float64 then mostly take different
+    shapes, and the kernel's mode counts are checked on the device against the mode geometry.  On
+    white noise without a window the mode counts agree exactly and every
     3-D, phase and 1-D bin to 1e-5; a 2-D bin holds a few modes, some of whose powers lie far below the
     mean, and float32 rounding moves a mode's power by an amount proportional to its amplitude times
     the field's, so 2-D bins are held to 1e-5 of the larger of their own and the mean 2-D power."""
     gen = torch.Generator(device="cuda")
     gen.manual_seed(N)
     d32 = torch.randn((N, N, N), device="cuda", generator=gen)
-    p32 = PKL.Pk(d32, 1000.0, 0, "None")
+    p32 = PKL.Pk(d32, 1000.0, axis, "None")
     d64 = d32.double()
     del d32
-    p64 = PKL.Pk(d64, 1000.0, 0, "None")
+    p64 = PKL.Pk(d64, 1000.0, axis, "None")
     del d64
     torch.cuda.empty_cache()
     for name in ("Nmodes3D", "Nmodes1D", "Nmodes2D"):
