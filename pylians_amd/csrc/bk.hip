@@ -220,7 +220,8 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
         if (it != g_isums.end()) isums = it->second;
     }
     const bool need_i = isums.empty();
-    rc = fft3d_r2c<R>(delta, spec, N, P, st);
+    // every later read of the spectra (the shell passes, the mode sums) is of columns kz < K
+    rc = fft3d_r2c<R>(delta, spec, N, P, st, K);
     if (rc) return rc;
     // shell k1: d1 (and I1); shell k2: store d1*d2 (and I1*I2)
     if ((rc = shell_field<R>(spec, ws, N, P, K, kmin[0], kmax[0], 0, win, d1, 1, nullptr, acc + 0, st))) return rc;
@@ -233,11 +234,11 @@ static int bk_run(const R* delta, int N, double BoxSize, double k1, double k2, c
     if (nbin) {
         cplx<R>* pd = ws;   // the shell transforms are done with it
         cplx<R>* pi = nullptr;
-        if ((rc = fft3d_r2c<R>(d12, pd, N, P, st))) return rc;
+        if ((rc = fft3d_r2c<R>(d12, pd, N, P, st, K))) return rc;
         if (need_i) {
             pi = (cplx<R>*)workspace(WS_AUX3, sizeof(cplx<R>) * nspec, &rc);
             if (rc) return rc;
-            if ((rc = fft3d_r2c<R>(I12, pi, N, P, st))) return rc;
+            if ((rc = fft3d_r2c<R>(I12, pi, N, P, st, K))) return rc;
         }
         double* bins = acc + nacc;
         const long long total = (long long)N * N * K;

@@ -474,8 +474,9 @@ static int cols_job_n(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, R s
     using SH = ColShape<R, N>;
     const cplx<R>* tw = twiddles<R>(N);
     PYL_CHECK(tw, E_ALLOC, "twiddle table allocation failed");
-    // K < W (pass along y only): just the columns kz < K, enumerated without the others
-    const bool lim = axis == 1 && K > 0 && K < N / 2 + 1;
+    // K < W: just the columns kz < K, enumerated without the others (column c = u * K + kz of the
+    // outer coordinate u: x for the pass along y, y for the pass along x)
+    const bool lim = K > 0 && K < N / 2 + 1;
     const long long PW = lim ? K : P;
     const int WV = lim ? K : N / 2 + 1;
     const long long ncols = (long long)N * PW;
@@ -484,6 +485,10 @@ static int cols_job_n(const cplx<R>* src, cplx<R>* dst, int64_t P, int axis, R s
         CW = PW;
         CS = (long long)N * P;
         EST = P;
+    } else if (lim) {
+        CW = PW;
+        CS = P;
+        EST = (long long)N * P;
     } else {
         CW = ncols;
         CS = 0;
@@ -572,12 +577,12 @@ int fft_pass_z_c2r(const cplx<R>* in, R* out, int N, int64_t P, R sc, int store,
 }
 
 template <typename R>
-int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st) {
+int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st, int K) {
     int rc = fft_pass_z_r2c<R>(in, out, N, P, false, nullptr, 1.0, st);
     if (rc) return rc;
-    rc = fft_pass_cols<R>(out, out, N, P, 1, false, (R)1, st);
+    rc = fft_pass_cols<R>(out, out, N, P, 1, false, (R)1, st, K);
     if (rc) return rc;
-    return fft_pass_cols<R>(out, out, N, P, 0, false, (R)1, st);
+    return fft_pass_cols<R>(out, out, N, P, 0, false, (R)1, st, K);
 }
 
 template <typename R>
@@ -650,7 +655,7 @@ int cols_batch(int N, const ColArgs<R>* tab, const unsigned* start, int nj, unsi
     template int fft_pass_z_r2c<R>(const R*, cplx<R>*, int, int64_t, bool, const double*, double, hipStream_t);               \
     template int fft_pass_cols<R>(const cplx<R>*, cplx<R>*, int, int64_t, int, bool, R, hipStream_t, int);    \
     template int fft_pass_z_c2r<R>(const cplx<R>*, R*, int, int64_t, R, int, const R*, double*, hipStream_t, int); \
-    template int fft3d_r2c<R>(const R*, cplx<R>*, int, int64_t, hipStream_t);                                \
+    template int fft3d_r2c<R>(const R*, cplx<R>*, int, int64_t, hipStream_t, int);                                \
     template int fft3d_c2r<R>(const cplx<R>*, R*, cplx<R>*, int, int64_t, R, hipStream_t);                   \
     template int r2c_job<R>(const R*, cplx<R>*, int, int64_t, bool, const double*, double, R2cArgs<R>*, unsigned*); \
     template int r2c_batch<R>(int, const R2cArgs<R>*, const unsigned*, int, unsigned, hipStream_t);          \

@@ -22,8 +22,8 @@ template <typename R>
 int fft_pass_z_r2c(const R* in, cplx<R>* out, int N, int64_t P, bool affine, const double* dmass, double mass,
                    hipStream_t st);
 
-// column pass along y (axis 1) or x (axis 0) on a half spectrum, src -> dst (may alias).  K > 0
-// (axis 1 only): transform just the columns kz < K and leave the others untouched.
+// column pass along y (axis 1) or x (axis 0) on a half spectrum, src -> dst (may alias).  K > 0:
+// transform just the columns kz < K and leave the others untouched.
 template <typename R>
 int fft_pass_cols(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, bool inverse, R scale,
                   hipStream_t st, int K = 0);
@@ -69,9 +69,10 @@ int cols_job(const cplx<R>* src, cplx<R>* dst, int N, int64_t P, int axis, ColAr
 template <typename R>
 int cols_batch(int N, const ColArgs<R>* tab, const unsigned* start, int nj, unsigned total, hipStream_t st);
 
-// full transforms
+// full transforms; with K > 0 only the modes kz < K of the r2c are complete (the column passes
+// skip the others, which keep the pass-Z values)
 template <typename R>
-int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st);
+int fft3d_r2c(const R* in, cplx<R>* out, int N, int64_t P, hipStream_t st, int K = 0);
 // in is not modified; ws must hold N*N*P complex values (may equal nothing else)
 template <typename R>
 int fft3d_c2r(const cplx<R>* in, R* out, cplx<R>* ws, int N, int64_t P, R scale, hipStream_t st);
