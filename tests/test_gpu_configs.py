@@ -145,13 +145,24 @@ def test_bk_zeldovich512_grid256_20theta(gpu):
     grid = torch.zeros((N, N, N), dtype=torch.float32, device="cuda")
     _lib.check(_lib.lib().pyl_mas_deposit(_lib.ptr(pos), None, int(pos.shape[0]), 3, N, L, 2, _lib.ZERO,
                                           _lib.ptr(grid), _lib.stream()))
+    # the benchmark's deposit: delta straight from pyl_mas_overdensity (8 particles per cell: the
+    # bucketing pass counts runs of equal keys once)
+    dgrid = torch.empty((N, N, N), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib().pyl_mas_overdensity(_lib.ptr(pos), None, int(pos.shape[0]), N, L, 2, 0, _lib.ptr(dgrid),
+                                              _lib.stream()))
+    n_p = int(pos.shape[0])
     del pos
+    dover = dgrid.cpu().numpy().astype(np.float64)
+    del dgrid
     grid /= grid.double().mean().float()
     grid -= 1.0
     theta = np.linspace(0, np.pi, 20)
     g = Pk_library.Bk(grid, L, 0.1, 0.2, theta, MAS="CIC")
     del grid
     rho = C.deposit_f64(host, N, L, "CIC")
+    exact = rho * (N ** 3 / n_p) - 1.0
+    assert np.all(np.abs(dover - exact) <= 6.5e-8 * np.abs(exact) + 1e-12)
+    del dover, exact
     delta = rho / rho.mean() - 1.0
     del rho, host
     o = Bo.Bk(delta, L, 0.1, 0.2, theta, MAS="CIC")
